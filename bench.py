@@ -1,0 +1,161 @@
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+Workload: the random-spheres scene (C2: ground + 22x22 jittered grid of
+r=0.2 spheres + three r=1 spheres), 1920x1080, 64 samples per pixel, depth 16,
+seed 0. One step = one PathTracer.render() of 64 spp over the whole image
+(wgpu_backend/path_tracer.zig:252-265), i.e. 132,710,400 samples, with the
+scene already resident in HBM. Metric: samples/s = W*H*spp*steps / wall time.
+
+Also reported:
+  roofline     algorithmic bytes per render (the reference algorithm's node,
+               transform, material and attribute fetches, counted on the same
+               paths by the instrumented CPU oracle in the cpu_baseline leg)
+               divided by the render kernel's average duration (HIP events on
+               the stream the kernel runs on), against 8 TB/s of HBM3E.
+  cpu_baseline the oracle (a C restatement of the WGSL megakernel, OpenMP
+               over pixels, serial samples per pixel) timed on the host on a
+               bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import zig_ornament_amd as zo  # noqa: E402
+from zig_ornament_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(stats):
+    """Bytes the reference algorithm fetches (per the counters of one run):
+    64 B per BVH node and per transform fetched during traversal; per shaded
+    hit 32 B of material, +64 B model transform for spheres, +160 B of
+    indices/normals/uvs/normal transform for meshes, +4 B per texel; 40 B per
+    sample of RNG state and accumulation read+write."""
+    hits = stats["sphere_hits"] + stats["mesh_hits"]
+    return (64 * stats["node_visits"] + 64 * stats["transform_fetches"] + 32 * hits
+            + 64 * stats["sphere_hits"] + (12 + 48 + 12 + 24 + 64) * stats["mesh_hits"]
+            + 4 * stats["textured_hits"] + 40 * stats["samples"])
+
+
+def cpu_baseline(spec, arrays, params, seconds=12.0):
+    """Times the oracle on a bounded, uniformly spread subset of the pixels at
+    full spp; returns samples/s, the sample size, threads and per-sample
+    algorithmic bytes of those paths."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    O.build()
+    threads = min(16, os.cpu_count() or 1)
+    n_px = spec.width * spec.height
+    sc = O.OracleScene(arrays)
+    p = O.ConstantParams.from_buffer_copy(bytes(params))
+    count = 2048
+    while True:
+        pix = np.linspace(0, n_px - 1, count).astype(np.uint32)
+        acc = np.zeros((count, 4), np.float32)
+        rng = pix.copy()
+        st = O.Stats()
+        t0 = time.perf_counter()
+        sc.render_pixels(p, spec.spp, pix, acc, rng, nthreads=threads, stats=st)
+        dt = time.perf_counter() - t0
+        if dt >= seconds or count >= n_px:
+            break
+        count = min(n_px, max(count * 2, int(count * seconds / max(dt, 1e-3) * 1.05)))
+    stats = {f: getattr(st, f) for f, _ in O.Stats._fields_}
+    return dict(samples_per_s=count * spec.spp / dt, pixels=count, seconds=dt, threads=threads,
+                stats=stats, bytes_per_sample=algorithmic_bytes(stats) / stats["samples"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the path tracer shards by pixel tiles with no "
+              "communication during rendering, and this build runs on a single MI355X. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    spec = W.random_spheres()  # 1920x1080, 64 spp, depth 16
+    t0 = time.perf_counter()
+    scene = W.build_scene(spec)
+    arrays = zo.Bvh(scene)
+    build_s = time.perf_counter() - t0
+    dev = zo.Device(0)
+    name, cus = dev.info()
+    ds = zo.DeviceScene(dev, arrays)
+    tgt = zo.Target(ds, spec.width, spec.height, 0)
+    params = zo._native.ConstantParams()
+    params.camera = scene.camera()
+    params.depth, params.width, params.height = spec.depth, spec.width, spec.height
+    params.flip_y, params.inverted_gamma = 0, 1.0
+    params.ray_cast_epsilon = spec.ray_cast_epsilon
+    params.textures_count = len(spec.textures)
+    params.current_iteration = 0.0
+
+    for _ in range(args.warmup):
+        tgt.render(params, spec.spp)
+    tgt.wait()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tgt.render(params, spec.spp)  # blocks until the GPU is done
+        kernel_ms.append(tgt.last_render_ms())
+    tgt.wait()
+    wall = time.perf_counter() - t0
+    samples = spec.samples * args.steps
+    value = samples / wall
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+
+    out = {
+        "metric": "samples/s (plain path tracing, random-spheres 1920x1080 64spp depth 16)",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene generator)",
+        "config": {"workload": "random_spheres_1920x1080_64spp_depth16", "width": spec.width,
+                   "height": spec.height, "spp": spec.spp, "depth": spec.depth,
+                   "spheres": len(spec.spheres), "tlas_nodes": int(len(arrays.tlas_nodes)),
+                   "parallelism": "single GPU, pixel-parallel", "device": name,
+                   "compute_units": cus, "bvh_build_s": build_s},
+        "kernel_ms_avg": avg_kernel_s * 1e3,
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(spec, arrays, params, args.cpu_seconds)
+        bytes_per_launch = cb["bytes_per_sample"] * spec.samples + 16 * spec.width * spec.height
+        achieved = bytes_per_launch / avg_kernel_s / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "algorithmic_bytes_per_sample": cb["bytes_per_sample"],
+                           "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
+                           "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"]}
+        out["cpu_baseline"] = {"value": cb["samples_per_s"], "unit": "samples/s",
+                               "cores": cb["threads"], "kind": "port",
+                               "sample": f"{cb['pixels']} evenly spaced pixels of the same image "
+                                         f"at 64 spp ({cb['pixels'] * spec.spp} samples, "
+                                         f"{cb['seconds']:.1f} s) on {cb['threads']} host threads"}
+        out["gpu_over_cpu"] = value / cb["samples_per_s"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

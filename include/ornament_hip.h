@@ -1,0 +1,248 @@
+/*
+ * ornament_hip.h -- the thin C-ABI between the ornament host layer (scene,
+ * camera, state, BVH build) and the MI355X (gfx950) HIP kernels.
+ *
+ * This is the boundary a Zig `HipMi355xPathTracer` binds with @cImport in
+ * place of the reference's WebGPU driver (src/wgpu_backend/path_tracer.zig)
+ * and HIP driver (src/hip_backend/path_tracer.zig). Plain C: opaque handles,
+ * plain pointers and sizes, every function returns an orn_status.
+ *
+ * Status codes: 0 is success; runtime failures carry the hipError_t value
+ * they came from (the numbering of CuintHipError, src/hip_backend/hip.zig:7-62,
+ * e.g. 1 InvalidValue, 2 OutOfMemory, 100 NoDevice); codes >= 9000 are
+ * ornament's own (an empty scene panics in the reference, bvh.zig:35-37).
+ *
+ * Threading: one handle may be used by one host thread at a time. Calls are
+ * synchronous unless their name ends in _async, like the reference, which
+ * blocks on wgpuDevicePoll(wait=true) after every dispatch
+ * (wgpu_backend/path_tracer.zig:248-249).
+ */
+#ifndef ORNAMENT_HIP_H
+#define ORNAMENT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+typedef int orn_status;
+#define ORN_OK 0
+#define ORN_ERROR_INVALID_VALUE 1          /* hipErrorInvalidValue */
+#define ORN_ERROR_OUT_OF_MEMORY 2          /* hipErrorOutOfMemory */
+#define ORN_ERROR_NO_DEVICE 100            /* hipErrorNoDevice */
+#define ORN_ERROR_INVALID_DEVICE 101       /* hipErrorInvalidDevice */
+#define ORN_ERROR_EMPTY_SCENE 9001         /* bvh.zig:35-37 panics */
+#define ORN_ERROR_EMPTY_BVH 9002           /* bvh.zig:222-223, 273-274 panic */
+#define ORN_ERROR_INVALID_SCENE 9003       /* an index in the arrays is out of range */
+#define ORN_ERROR_BVH_TOO_DEEP 9004        /* traversal stack > 64 (pathtracer.wgsl:20) */
+#define ORN_ERROR_UNSUPPORTED_TEXTURE 9005 /* wgpu buffers.zig:316-336 asserts */
+#define ORN_ERROR_INVALID_HANDLE 9006      /* null or destroyed handle */
+#define ORN_ERROR_SIZE_MISMATCH 9007       /* destination smaller than the target */
+
+/* Human-readable name of a status ("InvalidValue", "EmptyScene", ...). */
+const char *orn_status_string(orn_status status);
+
+/* ------------------------------------------------------------------ */
+/* Binary layouts: identical to the reference's extern structs          */
+/* ------------------------------------------------------------------ */
+
+/* BvhNode, gpu_structs.zig:22-31 (64 bytes). node_type: 0 internal,
+ * 1 sphere, 2 mesh, 3 triangle (gpu_structs.zig:15-20). */
+typedef struct orn_bvh_node {
+    float lmin_or_v0[3];
+    uint32_t left_or_custom_id;
+    float lmax_or_v1[3];
+    uint32_t right_or_material_index;
+    float rmin_or_v2[3];
+    uint32_t node_type;
+    float rmax_or_v3[3];
+    uint32_t transform_id;
+} orn_bvh_node;
+
+/* Material, gpu_structs.zig:33-59 (32 bytes). type: 0 Lambertian, 1 Metal,
+ * 2 Dielectric, 3 DiffuseLight (material.zig:4-9). */
+typedef struct orn_material {
+    float albedo[3];
+    uint32_t albedo_texture_index; /* 0xFFFFFFFF: plain colour */
+    float fuzz;
+    float ior;
+    uint32_t type;
+    uint32_t padding;
+} orn_material;
+
+/* Camera, gpu_structs.zig:88-117 (112 bytes) */
+typedef struct orn_camera {
+    float origin[3];
+    float lens_radius;
+    float lower_left_corner[3];
+    uint32_t padding0;
+    float horizontal[3];
+    uint32_t padding1;
+    float vertical[3];
+    uint32_t padding2;
+    float u[3];
+    uint32_t padding3;
+    float v[3];
+    uint32_t padding4;
+    float w[3];
+    uint32_t padding5;
+} orn_camera;
+
+/* ConstantParams, gpu_structs.zig:61-86 (144 bytes) */
+typedef struct orn_constant_params {
+    orn_camera camera;
+    uint32_t depth;
+    uint32_t width;
+    uint32_t height;
+    uint32_t flip_y;
+    float inverted_gamma;
+    float ray_cast_epsilon;
+    uint32_t textures_count;
+    float current_iteration;
+} orn_constant_params;
+
+/* One texture as ornament.Texture holds it (texture.zig:4-38). Supported
+ * formats (wgpu_backend/buffers.zig:316-336): 8-bit unorm with 1, 2 or 4
+ * components, and 16-bit float (is_hdr) with 1, 2 or 4 components. */
+typedef struct orn_texture_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t num_components;
+    uint32_t bytes_per_component;
+    uint32_t is_hdr;
+    uint32_t padding;
+    const uint8_t *data; /* width * height * num_components * bytes_per_component */
+} orn_texture_desc;
+
+/* The arrays Bvh.init produces (bvh.zig:22-30), as the WGSL megakernel binds
+ * them (pathtracer.wgsl:7-17). The library copies everything; the caller
+ * keeps ownership of the host arrays. */
+typedef struct orn_scene_desc {
+    const orn_bvh_node *tlas_nodes;
+    size_t tlas_nodes_len;
+    const orn_bvh_node *blas_nodes;
+    size_t blas_nodes_len;
+    const float *normals; /* 4 floats per normal (Normal, gpu_structs.zig:8) */
+    size_t normals_len;
+    const uint32_t *normal_indices;
+    size_t normal_indices_len;
+    const float *uvs; /* 2 floats per uv */
+    size_t uvs_len;
+    const uint32_t *uv_indices;
+    size_t uv_indices_len;
+    const float *transforms; /* 16 floats per matrix: [inverse, model] per leaf */
+    size_t transforms_len;
+    const orn_material *materials;
+    size_t materials_len;
+    const orn_texture_desc *textures;
+    size_t textures_len;
+} orn_scene_desc;
+
+/* Closest hit of one ray, the outputs of bvh_hit (bvh.wgsl:14-21). */
+typedef struct orn_hit {
+    uint32_t hit;
+    float t;
+    uint32_t material_index;
+    uint32_t node_type; /* 1 sphere, 2 mesh */
+    uint32_t inverted_transform_id;
+    uint32_t tri_id; /* 3 * global triangle index */
+    float uv[2];
+} orn_hit;
+
+typedef struct orn_ctx orn_ctx;
+typedef struct orn_device_scene orn_device_scene;
+typedef struct orn_target orn_target;
+
+/* ------------------------------------------------------------------ */
+/* Device                                                              */
+/* ------------------------------------------------------------------ */
+
+/* Number of visible HIP devices. */
+orn_status orn_device_count(int *count);
+
+/* Open a device and its stream. device < 0 selects the last device, as the
+ * reference's HIP driver does (hip_backend/path_tracer.zig:40-41). Replaces
+ * DeviceState.init (wgpu_backend/device_state.zig:18-92). */
+orn_status orn_device_open(int device, orn_ctx **out);
+orn_status orn_device_close(orn_ctx *ctx);
+/* Device name and compute-unit count, for logs (hip.zig:239-252). */
+orn_status orn_device_info(orn_ctx *ctx, char *name, size_t name_len, int *compute_units);
+
+/* ------------------------------------------------------------------ */
+/* Scene upload: replaces buffers.Storage/Textures.init                 */
+/* (wgpu_backend/path_tracer.zig:55-66, buffers.zig:113-336)            */
+/* ------------------------------------------------------------------ */
+
+/* Validates every index (node links, leaf transform/material ids, normal and
+ * uv indices, texture ids) so that no kernel can read out of bounds, then
+ * copies the arrays to HBM. Empty arrays are allowed except the TLAS. */
+orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device_scene **out);
+/* The validation orn_scene_upload performs, without a device: returns the
+ * status upload would return for these arrays and, on success, the
+ * traversal stack slots the BVH needs. */
+orn_status orn_scene_validate(const orn_scene_desc *desc, uint32_t *stack_slots);
+orn_status orn_scene_destroy(orn_device_scene *scene);
+/* Traversal stack slots the uploaded BVH needs (TLAS depth + BLAS depth + 3). */
+orn_status orn_scene_stack_depth(orn_device_scene *scene, uint32_t *depth);
+
+/* ------------------------------------------------------------------ */
+/* Render target: replaces buffers.Target (wgpu_backend/buffers.zig:11-98) */
+/* ------------------------------------------------------------------ */
+
+/* Framebuffer and accumulation (float4 per pixel, zero-initialised) and the
+ * per-pixel RNG state, seeded index + seed * width * height (seed 0 is the
+ * reference's seeding, buffers.zig:25-30). */
+orn_status orn_target_create(orn_device_scene *scene, uint32_t width, uint32_t height,
+                             uint32_t seed, orn_target **out);
+orn_status orn_target_destroy(orn_target *target);
+
+/* One PathTracer.render() (wgpu_backend/path_tracer.zig:252-265): `params`
+ * carries State.current_iteration as it is BEFORE the call; the kernel
+ * advances it by max(iterations, 1), accumulates one sample per pixel per
+ * iteration and writes the post-processed framebuffer once at the end.
+ * Blocks until the GPU is done. */
+orn_status orn_render(orn_target *target, const orn_constant_params *params, uint32_t iterations);
+/* Same, without waiting; orn_target_wait blocks until it has finished. */
+orn_status orn_render_async(orn_target *target, const orn_constant_params *params,
+                            uint32_t iterations);
+orn_status orn_target_wait(orn_target *target);
+/* GPU time of the most recent render's kernels (HIP events on the target's
+ * stream), in milliseconds. */
+orn_status orn_target_last_render_ms(orn_target *target, float *ms);
+
+/* Readback (Target.getFrameBuffer, wgpu_backend/buffers.zig:67-91):
+ * `pixels` is the number of float4 entries dst can hold. */
+orn_status orn_get_framebuffer(orn_target *target, float *dst, size_t pixels);
+orn_status orn_get_accumulation(orn_target *target, float *dst, size_t pixels);
+orn_status orn_get_rng_state(orn_target *target, uint32_t *dst, size_t pixels);
+/* Restore a saved accumulation + RNG state (checkpoint / resume of a
+ * progressive render; either pointer may be NULL to keep the current one). */
+orn_status orn_set_state(orn_target *target, const float *accumulation, const uint32_t *rng,
+                         size_t pixels);
+
+/* ------------------------------------------------------------------ */
+/* Ray queries                                                         */
+/* ------------------------------------------------------------------ */
+
+/* Closest hit of n world-space rays (6 floats each: origin, direction) with
+ * t_min = ray_cast_epsilon, by the same traversal the render kernel runs
+ * (bvh_hit, bvh.wgsl:14-137). Host arrays in and out; blocks. */
+orn_status orn_trace_rays(orn_device_scene *scene, float t_min, const float *rays, size_t n,
+                          orn_hit *out);
+
+/* Evaluates the device's FP32 transcendental functions (the kernels' stand-ins
+ * for the WGSL builtins pow, sin, cos, acos, atan2) over host arrays:
+ * op 0 sin(a), 1 cos(a), 2 pow(a, b), 3 acos(a), 4 atan2(a, b). */
+orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, float *out,
+                         size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,69 @@
+"""GPU at the benchmark sizes (random spheres 1920x1080 64 spp depth 16;
+Cornell 256x256 16 spp; instanced icospheres 1920x1080 32 spp): properties
+that do not need a full CPU reference, plus bit-exact parity with the oracle
+on a seeded subset of pixels rendered at full spp."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import zig_ornament_amd as zo
+from helpers import zo_params
+from zig_ornament_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def check_subset(prod, spec, acc_g, rng_g, pixels):
+    n = len(pixels)
+    acc = np.zeros((n, 4), np.float32)
+    rng = pixels.astype(np.uint32).copy()
+    p = O.ConstantParams.from_buffer_copy(bytes(zo_params(spec)))
+    O.OracleScene(prod).render_pixels(p, spec.spp, pixels, acc, rng)
+    assert acc.tobytes() == acc_g[pixels].tobytes()
+    assert rng.tobytes() == rng_g[pixels].tobytes()
+
+
+@pytest.mark.parametrize("name", ["C2", "C3"])
+def test_full_size_render(device, name):
+    spec = W.random_spheres() if name == "C2" else W.instanced_icospheres()
+    print(f"[full] {name} {spec.width}x{spec.height} {spec.spp} spp depth {spec.depth}", flush=True)
+    prod = zo.Bvh(W.build_scene(spec))
+    ds = zo.DeviceScene(device, prod)
+    a = zo.Target(ds, spec.width, spec.height)
+    a.render(zo_params(spec), spec.spp)
+    acc_a, rng_a = a.accumulation(), a.rng_state()
+    print(f"[full] {name} render {a.last_render_ms():.1f} ms", flush=True)
+    # every pixel accumulated exactly spp samples, all finite
+    assert np.all(acc_a[:, 3] == spec.spp)
+    assert np.all(np.isfinite(acc_a))
+    # determinism and progressive equivalence: spp/2 + spp/2 == spp
+    b = zo.Target(ds, spec.width, spec.height)
+    b.render(zo_params(spec), spec.spp // 2)
+    b.render(zo_params(spec, current_iteration=spec.spp // 2), spec.spp // 2)
+    assert b.accumulation().tobytes() == acc_a.tobytes()
+    assert b.rng_state().tobytes() == rng_a.tobytes()
+    # bit-exact against the oracle on 1500 seeded pixels at full spp
+    pixels = np.sort(np.random.default_rng(0).choice(spec.width * spec.height, 1500,
+                                                     replace=False)).astype(np.uint32)
+    check_subset(prod, spec, acc_a, rng_a, pixels)
+
+
+def test_cornell_4096_spp_convergence(device):
+    """relMSE against a 4096-spp image falls like 1/spp (same scene, seed and
+    RNG stream: the low-spp images are prefixes of the reference)."""
+    spec = W.cornell()
+    prod = zo.Bvh(W.build_scene(spec))
+    ds = zo.DeviceScene(device, prod)
+    t = zo.Target(ds, spec.width, spec.height)
+    errs = {}
+    done = 0
+    ref_t = zo.Target(ds, spec.width, spec.height)
+    ref_t.render(zo_params(spec), 4096)
+    ref = ref_t.accumulation()[:, :3] / 4096.0
+    from helpers import rel_mse
+    for spp in [16, 64, 256]:
+        t.render(zo_params(spec, current_iteration=float(done)), spp - done)
+        done = spp
+        errs[spp] = rel_mse(t.accumulation()[:, :3] / spp, ref)
+    print(f"[conv] relMSE vs 4096 spp: {errs}", flush=True)
+    assert errs[64] < errs[16] / 2.5 and errs[256] < errs[64] / 2.5

@@ -1,0 +1,719 @@
+// orn_runtime.hip -- gfx950 kernels of the path-tracing hot path and the
+// implementation of the C-ABI in include/ornament_hip.h.
+//
+// Kernel map (reference -> here):
+//   main_render / main_post_processing / main (pathtracer.wgsl:23-62), looped
+//   over State.iterations by the host (wgpu_backend/path_tracer.zig:252-265)
+//     -> render_kernel: one launch per render() call. Each lane owns one
+//        pixel and runs all of its samples back to back (the per-pixel RNG
+//        chain is serial, random.wgsl:3-9), regenerating a new camera path as
+//        soon as the previous one ends so that lanes of a wave stay busy
+//        across samples; the framebuffer is post-processed at the end.
+//   bvh_hit (bvh.wgsl:14-137) -> trace_kernel (ray queries).
+//
+// Launch shape: 256-lane workgroups cover 16x16 pixel tiles (a wave is a
+// 16x4 strip, so its rays are coherent). Workgroup b runs on XCD b % 8; the
+// tile order is remapped so that each XCD walks one contiguous band of the
+// image, keeping the BVH nodes its rays touch in that XCD's own L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ornament_hip.h"
+#include "orn_trace.h"
+
+using namespace orn;
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kTile = 16;  // kTile * kTile == kBlock
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kMaxStack = 63;  // the reference's private stack has 64 slots
+
+struct KernelParams {
+    orn_constant_params p;
+    uint32_t iterations;
+    uint32_t tiles_x;
+    uint32_t tiles;
+    uint32_t tiles_per_xcd;
+};
+
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t per_xcd)
+{
+    return (b % kXcds) * per_xcd + b / kXcds;
+}
+
+// pathtracer.wgsl:64-79 + 95-161 + 81-93 for one pixel over all iterations
+__global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParams kp,
+                                                        float4 *__restrict__ acc_buf,
+                                                        uint32_t *__restrict__ rng_buf,
+                                                        float4 *__restrict__ fb)
+{
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t tile = tile_of_block(blockIdx.x, kp.tiles_per_xcd);
+    if (tile >= kp.tiles) return;
+    const orn_constant_params &p = kp.p;
+    const uint32_t x = (tile % kp.tiles_x) * kTile + (threadIdx.x % kTile);
+    const uint32_t y = (tile / kp.tiles_x) * kTile + (threadIdx.x / kTile);
+    if (x >= p.width || y >= p.height) return;
+    const uint32_t pix = y * p.width + x;
+    uint32_t *stack = lds_stack + threadIdx.x;
+
+    const orn_camera &cam = p.camera;
+    const f3 cam_origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+    const f3 cam_llc = mk(cam.lower_left_corner[0], cam.lower_left_corner[1], cam.lower_left_corner[2]);
+    const f3 cam_h = mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2]);
+    const f3 cam_v = mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]);
+    const f3 cam_u = mk(cam.u[0], cam.u[1], cam.u[2]);
+    const f3 cam_vv = mk(cam.v[0], cam.v[1], cam.v[2]);
+    const float inv_w_den = static_cast<float>(p.width - 1u);
+    const float inv_h_den = static_cast<float>(p.height - 1u);
+
+    uint32_t rng = rng_buf[pix];
+    float4 acc = acc_buf[pix];
+    float cur = p.current_iteration;
+    uint32_t left = kp.iterations > 1u ? kp.iterations : 1u;
+
+    Ray ray;
+    f3 thr;
+    uint32_t bounce = 0;
+    // camera ray of a new sample: 2 jitter draws + 2 lens draws
+    auto start_sample = [&]() {
+        const float u = (static_cast<float>(x) + rng_f32(rng)) / inv_w_den;
+        const float v = (static_cast<float>(y) + rng_f32(rng)) / inv_h_den;
+        const f3 disk = random_in_unit_disk(rng);
+        const f3 rd = cam.lens_radius * disk;
+        const f3 offset = rd.x * cam_u + rd.y * cam_vv;
+        ray.o = cam_origin + offset;
+        ray.d = ((cam_llc + u * cam_h) + v * cam_v - cam_origin) - offset;
+        thr = mk(1.0f, 1.0f, 1.0f);
+        bounce = 0;
+    };
+    start_sample();
+    for (;;) {
+        f3 color;
+        bool done = false;
+        if (bounce >= p.depth) {
+            color = thr;  // depth exhausted: the throughput is returned
+            done = true;
+        } else {
+            Hit h;
+            const bool hit = bvh_hit(sc, p.ray_cast_epsilon, ray, stack, kBlock, h);
+            bounce++;
+            if (!hit) {
+                const f3 ud = normalize(ray.d);
+                const float tt = 0.5f * (ud.y + 1.0f);
+                const float a = (1.0f - tt) * 1.0f;
+                color = thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f);
+                done = true;
+            } else {
+                Ray next;
+                f3 w;
+                if (shade(sc, p.textures_count, ray, h, rng, next, w)) {
+                    ray = next;
+                    thr = thr * w;
+                } else {
+                    color = thr * w;
+                    done = true;
+                }
+            }
+        }
+        if (done) {
+            cur = cur + 1.0f;  // State.nextIteration
+            if (cur > 1.0f) {
+                acc.x = acc.x + color.x;
+                acc.y = acc.y + color.y;
+                acc.z = acc.z + color.z;
+                acc.w = acc.w + 1.0f;
+            } else {
+                acc = make_float4(color.x, color.y, color.z, 1.0f);
+            }
+            if (--left == 0u) break;
+            start_sample();
+        }
+    }
+    acc_buf[pix] = acc;
+    rng_buf[pix] = rng;
+    // pathtracer.wgsl:81-93
+    float4 o;
+    o.x = fminf(fmaxf(acc.x / cur, 0.0f), 1.0f);
+    o.y = fminf(fmaxf(acc.y / cur, 0.0f), 1.0f);
+    o.z = fminf(fmaxf(acc.z / cur, 0.0f), 1.0f);
+    o.w = fminf(fmaxf(acc.w / cur, 0.0f), 1.0f);
+    o.x = pow_f(o.x, p.inverted_gamma);
+    o.y = pow_f(o.y, p.inverted_gamma);
+    o.z = pow_f(o.z, p.inverted_gamma);
+    const uint32_t dst = p.flip_y < 1u ? pix : p.width * (p.height - y - 1u) + x;
+    fb[dst] = o;
+}
+
+__global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
+                                                       const float *__restrict__ rays, uint32_t n,
+                                                       orn_hit *__restrict__ out)
+{
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Ray r;
+    r.o = mk(rays[6u * i], rays[6u * i + 1u], rays[6u * i + 2u]);
+    r.d = mk(rays[6u * i + 3u], rays[6u * i + 4u], rays[6u * i + 5u]);
+    Hit h{};
+    const bool hit = bvh_hit(sc, t_min, r, lds_stack + threadIdx.x, kBlock, h);
+    orn_hit o{};
+    o.hit = hit ? 1u : 0u;
+    if (hit) {
+        o.t = h.t;
+        o.material_index = h.material;
+        o.node_type = h.node_type;
+        o.inverted_transform_id = h.inv_tid;
+        o.tri_id = h.tri_id;
+        o.uv[0] = h.u;
+        o.uv[1] = h.v;
+    }
+    out[i] = o;
+}
+
+__global__ void math_kernel(int op, const float *a, const float *b, float *out, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float r;
+    switch (op) {
+    case 0: { float s, c; sin_cos(a[i], s, c); r = s; break; }
+    case 1: { float s, c; sin_cos(a[i], s, c); r = c; break; }
+    case 2: r = pow_f(a[i], b[i]); break;
+    case 3: r = acos_f(a[i]); break;
+    default: r = atan2_f(a[i], b[i]); break;
+    }
+    out[i] = r;
+}
+
+__global__ void seed_rng_kernel(uint32_t *rng, uint32_t n, uint32_t base)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rng[i] = i + base;
+}
+
+__global__ void zero_kernel(float4 *a, float4 *b, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host runtime
+// ===========================================================================
+
+struct orn_ctx {
+    int device;
+    hipStream_t stream;
+    char name[256];
+    int compute_units;
+};
+
+struct orn_device_scene {
+    orn_ctx *ctx;
+    DevScene ds;
+    std::vector<void *> allocations;
+    uint32_t stack_slots;
+};
+
+struct orn_target {
+    orn_device_scene *scene;
+    uint32_t width, height;
+    float4 *fb;
+    float4 *acc;
+    uint32_t *rng;
+    hipEvent_t ev_start, ev_stop;
+    bool timed;
+};
+
+namespace {
+
+orn_status from_hip(hipError_t e) { return e == hipSuccess ? ORN_OK : static_cast<orn_status>(e); }
+
+#define ORN_TRY(expr)                          \
+    do {                                       \
+        hipError_t e_ = (expr);                \
+        if (e_ != hipSuccess) return from_hip(e_); \
+    } while (0)
+
+orn_status set_device(orn_ctx *ctx) { return from_hip(hipSetDevice(ctx->device)); }
+
+template <typename T>
+orn_status upload(orn_device_scene *s, const T *src, size_t n, const T **dst)
+{
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);  // empty arrays padded to one element
+    void *p = nullptr;
+    ORN_TRY(hipMalloc(&p, bytes));
+    s->allocations.push_back(p);
+    if (n) {
+        ORN_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    } else {
+        ORN_TRY(hipMemset(p, 0, bytes));
+    }
+    *dst = static_cast<const T *>(p);
+    return ORN_OK;
+}
+
+// Depth (edges on the longest root-to-leaf path) of every node. Children
+// precede their parent in both arrays (bvh.zig appends children first), which
+// the validation enforces, so one pass in index order suffices.
+std::vector<uint32_t> node_depths(const orn_bvh_node *nodes, size_t n)
+{
+    std::vector<uint32_t> d(n, 0);
+    for (size_t i = 0; i < n; i++)
+        if (nodes[i].node_type == 0u)
+            d[i] = 1u + std::max(d[nodes[i].left_or_custom_id], d[nodes[i].right_or_material_index]);
+    return d;
+}
+
+orn_status validate(const orn_scene_desc *d, uint32_t *stack_slots)
+{
+    if (!d->tlas_nodes || d->tlas_nodes_len == 0) return ORN_ERROR_EMPTY_SCENE;
+    if (d->tlas_nodes_len > 0x7fffffffu || d->blas_nodes_len > 0x7fffffffu) return ORN_ERROR_INVALID_SCENE;
+    auto need = [](const void *p, size_t n) { return n == 0 || p != nullptr; };
+    if (!need(d->blas_nodes, d->blas_nodes_len) || !need(d->normals, d->normals_len) ||
+        !need(d->normal_indices, d->normal_indices_len) || !need(d->uvs, d->uvs_len) ||
+        !need(d->uv_indices, d->uv_indices_len) || !need(d->transforms, d->transforms_len) ||
+        !need(d->materials, d->materials_len) || !need(d->textures, d->textures_len))
+        return ORN_ERROR_INVALID_VALUE;
+    for (size_t i = 0; i < d->normal_indices_len; i++)
+        if (d->normal_indices[i] >= d->normals_len) return ORN_ERROR_INVALID_SCENE;
+    for (size_t i = 0; i < d->uv_indices_len; i++)
+        if (d->uv_indices[i] >= d->uvs_len) return ORN_ERROR_INVALID_SCENE;
+    auto leaf_ok = [&](const orn_bvh_node &nd) {
+        const uint64_t tid = nd.transform_id;
+        return 2u * tid + 1u < d->transforms_len && nd.right_or_material_index < d->materials_len;
+    };
+    std::vector<uint8_t> blas_root_used(d->blas_nodes_len, 0);
+    for (size_t i = 0; i < d->tlas_nodes_len; i++) {
+        const orn_bvh_node &nd = d->tlas_nodes[i];
+        switch (nd.node_type) {
+        case 0u:
+            if (nd.left_or_custom_id >= i || nd.right_or_material_index >= i) return ORN_ERROR_INVALID_SCENE;
+            break;
+        case 1u:
+            if (!leaf_ok(nd)) return ORN_ERROR_INVALID_SCENE;
+            break;
+        case 2u:
+            if (!leaf_ok(nd) || nd.left_or_custom_id >= d->blas_nodes_len) return ORN_ERROR_INVALID_SCENE;
+            blas_root_used[nd.left_or_custom_id] = 1;
+            break;
+        default:
+            return ORN_ERROR_INVALID_SCENE;  // a triangle or unknown node cannot sit in the TLAS
+        }
+    }
+    for (size_t i = 0; i < d->blas_nodes_len; i++) {
+        const orn_bvh_node &nd = d->blas_nodes[i];
+        if (nd.node_type == 0u) {
+            if (nd.left_or_custom_id >= i || nd.right_or_material_index >= i) return ORN_ERROR_INVALID_SCENE;
+        } else if (nd.node_type == 3u) {
+            const uint64_t base = 3ull * nd.left_or_custom_id;
+            if (base + 2u >= d->normal_indices_len || base + 2u >= d->uv_indices_len)
+                return ORN_ERROR_INVALID_SCENE;
+        } else {
+            return ORN_ERROR_INVALID_SCENE;  // no nested instancing
+        }
+    }
+    for (size_t i = 0; i < d->textures_len; i++) {
+        const orn_texture_desc &t = d->textures[i];
+        const bool comps_ok = t.num_components == 1 || t.num_components == 2 || t.num_components == 4;
+        const bool fmt_ok = t.is_hdr ? t.bytes_per_component == 2 : t.bytes_per_component == 1;
+        if (!comps_ok || !fmt_ok || t.width == 0 || t.height == 0) return ORN_ERROR_UNSUPPORTED_TEXTURE;
+        if (!t.data) return ORN_ERROR_INVALID_VALUE;
+    }
+    const std::vector<uint32_t> td = node_depths(d->tlas_nodes, d->tlas_nodes_len);
+    const std::vector<uint32_t> bd = node_depths(d->blas_nodes, d->blas_nodes_len);
+    uint32_t blas_depth = 0;
+    for (size_t i = 0; i < d->blas_nodes_len; i++)
+        if (blas_root_used[i]) blas_depth = std::max(blas_depth, bd[i]);
+    // pending entries: one per TLAS level + 2 children, the BLAS marker and
+    // root, one per BLAS level + 2 children
+    const uint32_t slots = td[d->tlas_nodes_len - 1] + blas_depth + 4u;
+    if (slots > kMaxStack) return ORN_ERROR_BVH_TOO_DEEP;
+    *stack_slots = slots;
+    return ORN_OK;
+}
+
+uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+extern "C" {
+
+const char *orn_status_string(orn_status s)
+{
+    switch (s) {
+    case ORN_OK: return "Success";
+    case ORN_ERROR_EMPTY_SCENE: return "EmptyScene";
+    case ORN_ERROR_EMPTY_BVH: return "EmptyBvh";
+    case ORN_ERROR_INVALID_SCENE: return "InvalidScene";
+    case ORN_ERROR_BVH_TOO_DEEP: return "BvhTooDeep";
+    case ORN_ERROR_UNSUPPORTED_TEXTURE: return "UnsupportedTexture";
+    case ORN_ERROR_INVALID_HANDLE: return "InvalidHandle";
+    case ORN_ERROR_SIZE_MISMATCH: return "SizeMismatch";
+    default: return hipGetErrorName(static_cast<hipError_t>(s));
+    }
+}
+
+orn_status orn_device_count(int *count)
+{
+    if (!count) return ORN_ERROR_INVALID_VALUE;
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e == hipErrorNoDevice) {
+        *count = 0;
+        return ORN_OK;
+    }
+    return from_hip(e);
+}
+
+orn_status orn_device_open(int device, orn_ctx **out)
+{
+    if (!out) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return ORN_ERROR_NO_DEVICE;
+    if (device < 0) device = n - 1;
+    if (device >= n) return ORN_ERROR_INVALID_DEVICE;
+    ORN_TRY(hipSetDevice(device));
+    orn_ctx *c = new (std::nothrow) orn_ctx();
+    if (!c) return ORN_ERROR_OUT_OF_MEMORY;
+    c->device = device;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return from_hip(e);
+    }
+    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    c->compute_units = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return from_hip(e);
+    }
+    *out = c;
+    return ORN_OK;
+}
+
+orn_status orn_device_close(orn_ctx *ctx)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    set_device(ctx);
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ORN_OK;
+}
+
+orn_status orn_device_info(orn_ctx *ctx, char *name, size_t name_len, int *compute_units)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    if (name && name_len) {
+        std::strncpy(name, ctx->name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (compute_units) *compute_units = ctx->compute_units;
+    return ORN_OK;
+}
+
+orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device_scene **out)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    if (!desc || !out) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    uint32_t slots = 0;
+    orn_status st = validate(desc, &slots);
+    if (st != ORN_OK) return st;
+    ORN_TRY(hipSetDevice(ctx->device));
+    orn_device_scene *s = new (std::nothrow) orn_device_scene();
+    if (!s) return ORN_ERROR_OUT_OF_MEMORY;
+    s->ctx = ctx;
+    s->stack_slots = slots;
+    DevScene &ds = s->ds;
+    auto fail = [&](orn_status e) {
+        orn_scene_destroy(s);
+        return e;
+    };
+    const float4 *f4 = nullptr;
+    if ((st = upload(s, reinterpret_cast<const float4 *>(desc->tlas_nodes), 4 * desc->tlas_nodes_len, &f4))) return fail(st);
+    ds.tlas = f4;
+    if ((st = upload(s, reinterpret_cast<const float4 *>(desc->blas_nodes), 4 * desc->blas_nodes_len, &f4))) return fail(st);
+    ds.blas = f4;
+    if ((st = upload(s, reinterpret_cast<const float4 *>(desc->normals), desc->normals_len, &f4))) return fail(st);
+    ds.normals = f4;
+    if ((st = upload(s, reinterpret_cast<const float4 *>(desc->transforms), 4 * desc->transforms_len, &f4))) return fail(st);
+    ds.transforms = f4;
+    const uint32_t *u32 = nullptr;
+    if ((st = upload(s, desc->normal_indices, desc->normal_indices_len, &u32))) return fail(st);
+    ds.normal_indices = u32;
+    if ((st = upload(s, desc->uv_indices, desc->uv_indices_len, &u32))) return fail(st);
+    ds.uv_indices = u32;
+    const float2 *f2 = nullptr;
+    if ((st = upload(s, reinterpret_cast<const float2 *>(desc->uvs), desc->uvs_len, &f2))) return fail(st);
+    ds.uvs = f2;
+    const uint4 *u4 = nullptr;
+    if ((st = upload(s, reinterpret_cast<const uint4 *>(desc->materials), 2 * desc->materials_len, &u4))) return fail(st);
+    ds.materials = u4;
+    // textures: one texel pool with 16-byte aligned offsets
+    std::vector<DevTexture> tex(desc->textures_len);
+    uint64_t pool = 0;
+    for (size_t i = 0; i < desc->textures_len; i++) {
+        const orn_texture_desc &t = desc->textures[i];
+        tex[i].offset = pool;
+        tex[i].width = t.width;
+        tex[i].height = t.height;
+        tex[i].components = t.num_components;
+        tex[i].hdr = t.is_hdr ? 1u : 0u;
+        const uint64_t bytes = uint64_t(t.width) * t.height * t.num_components * t.bytes_per_component;
+        pool += (bytes + 15u) & ~uint64_t(15);
+    }
+    std::vector<uint8_t> texels(pool);
+    for (size_t i = 0; i < desc->textures_len; i++) {
+        const orn_texture_desc &t = desc->textures[i];
+        const uint64_t bytes = uint64_t(t.width) * t.height * t.num_components * t.bytes_per_component;
+        std::memcpy(texels.data() + tex[i].offset, t.data, bytes);
+    }
+    const DevTexture *dt = nullptr;
+    if ((st = upload(s, tex.data(), tex.size(), &dt))) return fail(st);
+    ds.textures = dt;
+    const uint8_t *u8 = nullptr;
+    if ((st = upload(s, texels.data(), texels.size(), &u8))) return fail(st);
+    ds.texels = u8;
+    ds.tlas_len = static_cast<uint32_t>(desc->tlas_nodes_len);
+    ds.textures_len = static_cast<uint32_t>(desc->textures_len);
+    ds.stack_slots = slots;
+    *out = s;
+    return ORN_OK;
+}
+
+orn_status orn_scene_validate(const orn_scene_desc *desc, uint32_t *stack_slots)
+{
+    if (!desc) return ORN_ERROR_INVALID_VALUE;
+    uint32_t slots = 0;
+    orn_status st = validate(desc, &slots);
+    if (st == ORN_OK && stack_slots) *stack_slots = slots;
+    return st;
+}
+
+orn_status orn_scene_destroy(orn_device_scene *s)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    set_device(s->ctx);
+    for (void *p : s->allocations) hipFree(p);
+    delete s;
+    return ORN_OK;
+}
+
+orn_status orn_scene_stack_depth(orn_device_scene *s, uint32_t *depth)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    if (!depth) return ORN_ERROR_INVALID_VALUE;
+    *depth = s->stack_slots;
+    return ORN_OK;
+}
+
+orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t height, uint32_t seed,
+                             orn_target **out)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    if (!out || width == 0 || height == 0) return ORN_ERROR_INVALID_VALUE;
+    const uint64_t n = uint64_t(width) * height;
+    if (n > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    ORN_TRY(hipSetDevice(s->ctx->device));
+    orn_target *t = new (std::nothrow) orn_target();
+    if (!t) return ORN_ERROR_OUT_OF_MEMORY;
+    t->scene = s;
+    t->width = width;
+    t->height = height;
+    auto fail = [&](hipError_t e) {
+        orn_target_destroy(t);
+        return from_hip(e);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&t->fb, n * sizeof(float4))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&t->acc, n * sizeof(float4))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&t->rng, n * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipEventCreate(&t->ev_start)) != hipSuccess) return fail(e);
+    if ((e = hipEventCreate(&t->ev_stop)) != hipSuccess) return fail(e);
+    hipStream_t st = s->ctx->stream;
+    const uint32_t nn = static_cast<uint32_t>(n);
+    zero_kernel<<<blocks_for(nn, 256), 256, 0, st>>>(t->fb, t->acc, nn);
+    seed_rng_kernel<<<blocks_for(nn, 256), 256, 0, st>>>(t->rng, nn, seed * nn);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
+    *out = t;
+    return ORN_OK;
+}
+
+orn_status orn_target_destroy(orn_target *t)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    set_device(t->scene->ctx);
+    hipStreamSynchronize(t->scene->ctx->stream);
+    if (t->fb) hipFree(t->fb);
+    if (t->acc) hipFree(t->acc);
+    if (t->rng) hipFree(t->rng);
+    if (t->ev_start) hipEventDestroy(t->ev_start);
+    if (t->ev_stop) hipEventDestroy(t->ev_stop);
+    delete t;
+    return ORN_OK;
+}
+
+orn_status orn_render_async(orn_target *t, const orn_constant_params *params, uint32_t iterations)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    if (!params) return ORN_ERROR_INVALID_VALUE;
+    if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
+    orn_device_scene *s = t->scene;
+    ORN_TRY(hipSetDevice(s->ctx->device));
+    KernelParams kp;
+    kp.p = *params;
+    kp.iterations = iterations;
+    kp.tiles_x = blocks_for(t->width, kTile);
+    kp.tiles = kp.tiles_x * blocks_for(t->height, kTile);
+    kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
+    const uint32_t grid = kp.tiles_per_xcd * kXcds;
+    const size_t lds = size_t(s->stack_slots) * kBlock * sizeof(uint32_t);
+    hipStream_t st = s->ctx->stream;
+    ORN_TRY(hipEventRecord(t->ev_start, st));
+    render_kernel<<<grid, kBlock, lds, st>>>(s->ds, kp, t->acc, t->rng, t->fb);
+    ORN_TRY(hipGetLastError());
+    ORN_TRY(hipEventRecord(t->ev_stop, st));
+    t->timed = true;
+    return ORN_OK;
+}
+
+orn_status orn_target_wait(orn_target *t)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    ORN_TRY(hipSetDevice(t->scene->ctx->device));
+    ORN_TRY(hipStreamSynchronize(t->scene->ctx->stream));
+    return ORN_OK;
+}
+
+orn_status orn_render(orn_target *t, const orn_constant_params *params, uint32_t iterations)
+{
+    orn_status s = orn_render_async(t, params, iterations);
+    if (s != ORN_OK) return s;
+    return orn_target_wait(t);
+}
+
+orn_status orn_target_last_render_ms(orn_target *t, float *ms)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    if (!ms) return ORN_ERROR_INVALID_VALUE;
+    *ms = 0.0f;
+    if (!t->timed) return ORN_OK;
+    ORN_TRY(hipEventSynchronize(t->ev_stop));
+    ORN_TRY(hipEventElapsedTime(ms, t->ev_start, t->ev_stop));
+    return ORN_OK;
+}
+
+static orn_status readback(orn_target *t, void *dst, const void *src, size_t elem, size_t pixels)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    if (!dst) return ORN_ERROR_INVALID_VALUE;
+    const size_t n = size_t(t->width) * t->height;
+    if (pixels < n) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(t->scene->ctx->device));
+    hipStream_t st = t->scene->ctx->stream;
+    ORN_TRY(hipMemcpyAsync(dst, src, n * elem, hipMemcpyDeviceToHost, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+orn_status orn_get_framebuffer(orn_target *t, float *dst, size_t pixels)
+{
+    return readback(t, dst, t ? t->fb : nullptr, sizeof(float4), pixels);
+}
+
+orn_status orn_get_accumulation(orn_target *t, float *dst, size_t pixels)
+{
+    return readback(t, dst, t ? t->acc : nullptr, sizeof(float4), pixels);
+}
+
+orn_status orn_get_rng_state(orn_target *t, uint32_t *dst, size_t pixels)
+{
+    return readback(t, dst, t ? t->rng : nullptr, sizeof(uint32_t), pixels);
+}
+
+orn_status orn_set_state(orn_target *t, const float *accumulation, const uint32_t *rng, size_t pixels)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    const size_t n = size_t(t->width) * t->height;
+    if (pixels != n) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(t->scene->ctx->device));
+    hipStream_t st = t->scene->ctx->stream;
+    if (accumulation) ORN_TRY(hipMemcpyAsync(t->acc, accumulation, n * sizeof(float4), hipMemcpyHostToDevice, st));
+    if (rng) ORN_TRY(hipMemcpyAsync(t->rng, rng, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, size_t n, orn_hit *out)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    if ((!rays || !out) && n) return ORN_ERROR_INVALID_VALUE;
+    if (n == 0) return ORN_OK;
+    if (n > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
+    ORN_TRY(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    float *d_rays = nullptr;
+    orn_hit *d_out = nullptr;
+    hipError_t e = hipMalloc(&d_rays, n * 6 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&d_out, n * sizeof(orn_hit));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_rays, rays, n * 6 * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const size_t lds = size_t(s->stack_slots) * kBlock * sizeof(uint32_t);
+        trace_kernel<<<blocks_for(static_cast<uint32_t>(n), kBlock), kBlock, lds, st>>>(
+            s->ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(orn_hit), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipStreamSynchronize(st);
+    if (d_rays) hipFree(d_rays);
+    if (d_out) hipFree(d_out);
+    return from_hip(e);
+}
+
+orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, float *out, size_t n)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    if (op < 0 || op > 4 || !a || !out || ((op == 2 || op == 4) && !b)) return ORN_ERROR_INVALID_VALUE;
+    if (n == 0) return ORN_OK;
+    if (n > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
+    ORN_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    float *d = nullptr;
+    hipError_t e = hipMalloc(&d, 3 * n * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(d, a, n * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b) e = hipMemcpyAsync(d + n, b, n * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        math_kernel<<<blocks_for(static_cast<uint32_t>(n), 256), 256, 0, st>>>(op, d, d + n, d + 2 * n,
+                                                                              static_cast<uint32_t>(n));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipStreamSynchronize(st);
+    if (d) hipFree(d);
+    return from_hip(e);
+}
+
+}  // extern "C"

@@ -1,0 +1,432 @@
+// orn_trace.h -- device-side scene view, BVH traversal and shading for the
+// gfx950 path-tracing kernels.
+//
+// Semantics follow the reference's WGSL megakernel
+// (src/wgpu_backend/shaders/*.wgsl) operation for operation, so that the
+// closest hit, every RNG draw and every accumulated value are reproduced
+// bit for bit. The layout work is the MI355X part: nodes are fetched as four
+// 16-byte vector loads, the traversal stack lives in LDS (one column per
+// lane, stride = block size, so the 64 lanes of a wave hit 64 different
+// banks), and materials are two 16-byte loads.
+#pragma once
+
+#include "orn_common.h"
+
+namespace orn {
+
+constexpr uint32_t kFinishedTraverseBlas = 0xffffffffu;  // pathtracer.wgsl:19
+constexpr float kTMaxInit = 3.40282e+38f;                 // bvh.wgsl:23
+
+// One texture in the texel pool. format: components | (hdr << 8)
+struct DevTexture {
+    uint64_t offset;  // byte offset in the texel pool
+    uint32_t width;
+    uint32_t height;
+    uint32_t components;
+    uint32_t hdr;
+};
+
+// The bound buffers of pathtracer.wgsl:7-17, resident in HBM.
+struct DevScene {
+    const float4 *__restrict__ tlas;        // 4 float4 per node
+    const float4 *__restrict__ blas;        // 4 float4 per node
+    const float4 *__restrict__ normals;     // 1 float4 per normal
+    const uint32_t *__restrict__ normal_indices;
+    const float2 *__restrict__ uvs;
+    const uint32_t *__restrict__ uv_indices;
+    const float4 *__restrict__ transforms;  // 4 float4 (columns) per matrix
+    const uint4 *__restrict__ materials;    // 2 uint4 per material
+    const DevTexture *__restrict__ textures;
+    const uint8_t *__restrict__ texels;
+    uint32_t tlas_len;
+    uint32_t textures_len;
+    uint32_t stack_slots;  // LDS stack entries per lane
+    uint32_t pad;
+};
+
+struct Hit {
+    float t;
+    uint32_t material;
+    uint32_t node_type;
+    uint32_t inv_tid;  // 2 * transform_id
+    uint32_t tri_id;   // 3 * triangle index
+    float u, v;
+};
+
+struct Ray {
+    f3 o, d;
+};
+
+// bvh.wgsl:139-174
+__device__ __forceinline__ float copysign_ref(float a, float b) { return (b < 0.0f) ? -a : a; }
+
+__device__ __forceinline__ f3 safe_invdir(f3 d)
+{
+    const float eps = 1e-5f;
+    const float x = (fabsf(d.x) > eps) ? d.x : copysign_ref(eps, d.x);
+    const float y = (fabsf(d.y) > eps) ? d.y : copysign_ref(eps, d.y);
+    const float z = (fabsf(d.z) > eps) ? d.z : copysign_ref(eps, d.z);
+    return f3{1.0f / x, 1.0f / y, 1.0f / z};
+}
+
+// bvh.wgsl:184-192: slab test with fused multiply-adds
+__device__ __forceinline__ bool aabb_hit(float3 mn, float3 mx, f3 inv, f3 ox, float t_min,
+                                         float t_max)
+{
+    const float fx = fmaf(mx.x, inv.x, ox.x), fy = fmaf(mx.y, inv.y, ox.y),
+                fz = fmaf(mx.z, inv.z, ox.z);
+    const float nx = fmaf(mn.x, inv.x, ox.x), ny = fmaf(mn.y, inv.y, ox.y),
+                nz = fmaf(mn.z, inv.z, ox.z);
+    const float far_t = fminf(fminf(fminf(fmaxf(fx, nx), fmaxf(fy, ny)), fmaxf(fz, nz)), t_max);
+    const float near_t = fmaxf(fmaxf(fmaxf(fminf(fx, nx), fminf(fy, ny)), fminf(fz, nz)), t_min);
+    return near_t <= far_t;
+}
+
+// bvh.wgsl:194-226: Moller-Trumbore as the reference writes it (no
+// determinant test; NaN falls through the comparisons)
+__device__ __forceinline__ float triangle_hit(const Ray &r, f3 v1, f3 v2, f3 v3, float t_min,
+                                              float t_max, float &hu, float &hv)
+{
+    const f3 e1 = v2 - v1;
+    const f3 e2 = v3 - v1;
+    const f3 s1 = cross(r.d, e2);
+    const float invd = 1.0f / dot(s1, e1);
+    const f3 d = r.o - v1;
+    const float u = dot(d, s1) * invd;
+    if (u < 0.0f || u > 1.0f) return t_max;
+    const f3 s2 = cross(d, e1);
+    const float v = dot(r.d, s2) * invd;
+    if (v < 0.0f || u + v > 1.0f) return t_max;
+    const float t = dot(e2, s2) * invd;
+    if (t < t_min || t > t_max) return t_max;
+    hu = u;
+    hv = v;
+    return t;
+}
+
+// bvh.wgsl:228-249: unit sphere at the origin of object space
+__device__ __forceinline__ float sphere_hit(const Ray &r, float t_min, float t_max)
+{
+    const float a = r.d.x * r.d.x + r.d.y * r.d.y + r.d.z * r.d.z;
+    const float half_b = dot(r.o, r.d);
+    const float c = (r.o.x * r.o.x + r.o.y * r.o.y + r.o.z * r.o.z) - 1.0f;
+    const float disc = half_b * half_b - a * c;
+    if (disc < 0.0f) return t_max;
+    const float sq = sqrtf(disc);
+    float t = (-half_b - sq) / a;
+    if (t < t_min || t_max < t) {
+        t = (-half_b + sq) / a;
+        if (t < t_min || t_max < t) return t_max;
+    }
+    return t;
+}
+
+// transform.wgsl: the 16 floats are read column-major; c0..c3 are columns
+__device__ __forceinline__ f3 xform_point(const float4 *m, f3 p)
+{
+    const float4 c0 = m[0], c1 = m[1], c2 = m[2], c3 = m[3];
+    return f3{c0.x * p.x + c1.x * p.y + c2.x * p.z + c3.x * 1.0f,
+              c0.y * p.x + c1.y * p.y + c2.y * p.z + c3.y * 1.0f,
+              c0.z * p.x + c1.z * p.y + c2.z * p.z + c3.z * 1.0f};
+}
+
+__device__ __forceinline__ Ray xform_ray(const float4 *m, const Ray &r)
+{
+    const float4 c0 = m[0], c1 = m[1], c2 = m[2], c3 = m[3];
+    Ray o;
+    o.o = f3{c0.x * r.o.x + c1.x * r.o.y + c2.x * r.o.z + c3.x * 1.0f,
+             c0.y * r.o.x + c1.y * r.o.y + c2.y * r.o.z + c3.y * 1.0f,
+             c0.z * r.o.x + c1.z * r.o.y + c2.z * r.o.z + c3.z * 1.0f};
+    o.d = f3{c0.x * r.d.x + c1.x * r.d.y + c2.x * r.d.z + c3.x * 0.0f,
+             c0.y * r.d.x + c1.y * r.d.y + c2.y * r.d.z + c3.y * 0.0f,
+             c0.z * r.d.x + c1.z * r.d.y + c2.z * r.d.z + c3.z * 0.0f};
+    return o;
+}
+
+// transform.wgsl:16-19: transpose(M) * (n, 0), i.e. dot(column_i, (n, 0))
+__device__ __forceinline__ f3 xform_normal(const float4 *m, f3 n)
+{
+    const float4 c0 = m[0], c1 = m[1], c2 = m[2];
+    return f3{c0.x * n.x + c0.y * n.y + c0.z * n.z + c0.w * 0.0f,
+              c1.x * n.x + c1.y * n.y + c1.z * n.z + c1.w * 0.0f,
+              c2.x * n.x + c2.y * n.y + c2.z * n.z + c2.w * 0.0f};
+}
+
+// bvh.wgsl:14-137. `stack` is this lane's LDS column, `stride` the block size.
+// Same visiting order as the reference (left pushed before right, so the
+// right child is popped first) and the same strict `t < t_max` update, so
+// ties between equal distances resolve the same way.
+__device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const Ray &world,
+                                        uint32_t *stack, uint32_t stride, Hit &h)
+{
+    float t_max = kTMaxInit;
+    uint32_t top = 0;  // entries above the reference's bottom slot
+    uint32_t addr = sc.tlas_len - 1u;
+    bool in_tlas = true;
+    bool hit_anything = false;
+    Ray ray = world;
+    f3 inv = safe_invdir(ray.d);
+    f3 ox = (-ray.o) * inv;
+    const f3 inv_w = inv, ox_w = ox;
+    uint32_t material_index = 0u, inv_tid = 0u;
+    const uint32_t cap = sc.stack_slots;
+    for (;;) {
+        const float4 *node = (in_tlas ? sc.tlas : sc.blas) + 4u * addr;
+        const float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
+        const uint32_t type = as_u32(n2.w);
+        if (type == 0u) {
+            const bool l = aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z), inv,
+                                    ox, t_min, t_max);
+            const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z), inv,
+                                    ox, t_min, t_max);
+            if (l && top < cap) stack[(top++) * stride] = as_u32(n0.w);
+            if (r && top < cap) stack[(top++) * stride] = as_u32(n1.w);
+        } else if (type == 1u) {
+            const uint32_t tid = as_u32(n3.w) * 2u;
+            const float t = sphere_hit(xform_ray(sc.transforms + 4u * tid, ray), t_min, t_max);
+            if (t < t_max) {
+                hit_anything = true;
+                t_max = t;
+                h.t = t;
+                h.material = as_u32(n1.w);
+                h.node_type = 1u;
+                h.inv_tid = tid;
+            }
+        } else if (type == 2u) {
+            in_tlas = false;
+            if (top + 2u <= cap) {
+                stack[(top++) * stride] = kFinishedTraverseBlas;
+                stack[(top++) * stride] = as_u32(n0.w);
+            }
+            inv_tid = as_u32(n3.w) * 2u;
+            material_index = as_u32(n1.w);
+            ray = xform_ray(sc.transforms + 4u * inv_tid, ray);
+            inv = safe_invdir(ray.d);
+            ox = (-ray.o) * inv;
+        } else if (type == 3u) {
+            float u = 0.0f, v = 0.0f;
+            const float t = triangle_hit(ray, xyz(n0), xyz(n1), xyz(n2), t_min, t_max, u, v);
+            if (t < t_max) {
+                hit_anything = true;
+                t_max = t;
+                h.t = t;
+                h.material = material_index;
+                h.node_type = 2u;
+                h.inv_tid = inv_tid;
+                h.tri_id = as_u32(n0.w) * 3u;
+                h.u = u;
+                h.v = v;
+            }
+        }
+        if (top == 0u) break;
+        addr = stack[(--top) * stride];
+        if (addr == kFinishedTraverseBlas) {
+            in_tlas = true;
+            ray = world;
+            inv = inv_w;
+            ox = ox_w;
+            if (top == 0u) break;
+            addr = stack[(--top) * stride];
+        }
+    }
+    return hit_anything;
+}
+
+// ---------------------------------------------------------------------------
+// Shading: material.wgsl, hitrecord.wgsl, pathtracer.wgsl:95-161
+// ---------------------------------------------------------------------------
+
+struct Material {
+    f3 albedo;
+    uint32_t texture;
+    float fuzz;
+    float ior;
+    uint32_t type;
+};
+
+__device__ __forceinline__ Material load_material(const DevScene &sc, uint32_t i)
+{
+    const uint4 a = sc.materials[2u * i], b = sc.materials[2u * i + 1u];
+    Material m;
+    m.albedo = f3{as_f32(a.x), as_f32(a.y), as_f32(a.z)};
+    m.texture = a.w;
+    m.fuzz = as_f32(b.x);
+    m.ior = as_f32(b.y);
+    m.type = b.z;
+    return m;
+}
+
+__device__ __forceinline__ float half_to_float(uint32_t h)
+{
+    const uint32_t sign = (h >> 15) << 31;
+    const uint32_t e = (h >> 10) & 0x1fu, man = h & 0x3ffu;
+    if (e == 0u) {
+        const float f = static_cast<float>(man) * 5.9604644775390625e-08f;
+        return sign ? -f : f;
+    }
+    if (e == 31u) return as_f32(sign | 0x7f800000u | (man << 13));
+    return as_f32(sign | ((e + 112u) << 23) | (man << 13));
+}
+
+// textureSampleLevel with the default WebGPU sampler: nearest, clamp to edge,
+// level 0 (webgpu.zig:812-825); rgba8unorm reads as byte / 255.
+__device__ __forceinline__ f3 sample_texture(const DevScene &sc, uint32_t id, float u, float v)
+{
+    const DevTexture tx = sc.textures[id];
+    float fx = floorf(u * static_cast<float>(tx.width));
+    float fy = floorf(v * static_cast<float>(tx.height));
+    fx = fminf(fmaxf(fx, 0.0f), static_cast<float>(tx.width - 1u));
+    fy = fminf(fmaxf(fy, 0.0f), static_cast<float>(tx.height - 1u));
+    const uint64_t texel = static_cast<uint64_t>(static_cast<uint32_t>(fy)) * tx.width +
+                           static_cast<uint32_t>(fx);
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    if (tx.hdr) {
+        const uint16_t *d =
+            reinterpret_cast<const uint16_t *>(sc.texels + tx.offset) + texel * tx.components;
+        if (tx.components == 4u) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(d);
+            c[0] = half_to_float(q.x & 0xffffu);
+            c[1] = half_to_float(q.x >> 16);
+            c[2] = half_to_float(q.y & 0xffffu);
+        } else {
+            for (uint32_t k = 0; k < tx.components; k++) c[k] = half_to_float(d[k]);
+        }
+    } else {
+        const uint8_t *d = sc.texels + tx.offset + texel * tx.components;
+        if (tx.components == 4u) {
+            const uint32_t q = *reinterpret_cast<const uint32_t *>(d);
+            c[0] = static_cast<float>(q & 0xffu) / 255.0f;
+            c[1] = static_cast<float>((q >> 8) & 0xffu) / 255.0f;
+            c[2] = static_cast<float>((q >> 16) & 0xffu) / 255.0f;
+        } else {
+            for (uint32_t k = 0; k < tx.components; k++) c[k] = static_cast<float>(d[k]) / 255.0f;
+        }
+    }
+    return f3{c[0], c[1], c[2]};
+}
+
+// material.wgsl:91-99. Textures beyond what was uploaded read as "no texture"
+// (the reference would index its binding array out of range there).
+__device__ __forceinline__ f3 material_color(const DevScene &sc, const Material &m,
+                                             uint32_t textures_count, float u, float v)
+{
+    if (m.texture < textures_count && m.texture < sc.textures_len)
+        return sample_texture(sc, m.texture, u, v);
+    return m.albedo;
+}
+
+__device__ __forceinline__ bool near_zero(f3 e)
+{
+    return fabsf(e.x) < 1e-8f && fabsf(e.y) < 1e-8f && fabsf(e.z) < 1e-8f;
+}
+
+// WGSL reflect / refract
+__device__ __forceinline__ f3 reflect_dir(f3 v, f3 n) { return v - (2.0f * dot(n, v)) * n; }
+
+__device__ __forceinline__ f3 refract_dir(f3 e1, f3 e2, float eta)
+{
+    const float d = dot(e2, e1);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    return eta * e1 - (eta * d + sqrtf(k)) * e2;
+}
+
+// Shading of one closest hit: fills the next ray and the attenuation, or the
+// emitted radiance when the path ends (pathtracer.wgsl:114-157).
+// Returns true when the path continues.
+__device__ __forceinline__ bool shade(const DevScene &sc, uint32_t textures_count, const Ray &ray,
+                                      const Hit &h, uint32_t &rng, Ray &next, f3 &weight)
+{
+    const f3 p = ray.o + h.t * ray.d;
+    const Material m = load_material(sc, h.material);
+    const bool textured = m.texture < textures_count && m.texture < sc.textures_len;
+    f3 normal;
+    bool front;
+    float tu = 0.0f, tv = 0.0f;
+    f3 outward;
+    if (h.node_type == 1u) {
+        // sphere: outward = normalize(p - M * (0,0,0,1))
+        const float4 c3 = sc.transforms[4u * (h.inv_tid + 1u) + 3u];
+        const float4 c0 = sc.transforms[4u * (h.inv_tid + 1u)];
+        const float4 c1 = sc.transforms[4u * (h.inv_tid + 1u) + 1u];
+        const float4 c2 = sc.transforms[4u * (h.inv_tid + 1u) + 2u];
+        const f3 center = f3{c0.x * 0.0f + c1.x * 0.0f + c2.x * 0.0f + c3.x * 1.0f,
+                             c0.y * 0.0f + c1.y * 0.0f + c2.y * 0.0f + c3.y * 1.0f,
+                             c0.z * 0.0f + c1.z * 0.0f + c2.z * 0.0f + c3.z * 1.0f};
+        outward = normalize(p - center);
+        if (textured) {  // uv only reaches the result through a texture fetch
+            const float theta = acos_f(-outward.y);
+            const float phi = atan2_f(-outward.z, outward.x) + kPi;
+            tu = phi / (2.0f * kPi);
+            tv = theta / kPi;
+        }
+    } else {
+        // mesh: barycentric shading normal and uv, normal by transpose(inverse)
+        const uint32_t i0 = sc.normal_indices[h.tri_id], i1 = sc.normal_indices[h.tri_id + 1u],
+                       i2 = sc.normal_indices[h.tri_id + 2u];
+        const f3 n0 = xyz(sc.normals[i0]), n1 = xyz(sc.normals[i1]), n2 = xyz(sc.normals[i2]);
+        const float w = 1.0f - h.u - h.v;
+        const f3 nrm = (w * n0 + h.u * n1) + h.v * n2;
+        if (textured) {
+            const float2 uv0 = sc.uvs[sc.uv_indices[h.tri_id]];
+            const float2 uv1 = sc.uvs[sc.uv_indices[h.tri_id + 1u]];
+            const float2 uv2 = sc.uvs[sc.uv_indices[h.tri_id + 2u]];
+            tu = w * uv0.x + h.u * uv1.x + h.v * uv2.x;
+            tv = w * uv0.y + h.u * uv1.y + h.v * uv2.y;
+        }
+        outward = normalize(xform_normal(sc.transforms + 4u * h.inv_tid, nrm));
+    }
+    // hitrecord.wgsl:10-18
+    if (dot(ray.d, outward) > 0.0f) {
+        normal = -outward;
+        front = false;
+    } else {
+        normal = outward;
+        front = true;
+    }
+    switch (m.type) {
+    case 0u: {  // material.wgsl:9-20
+        f3 dir = normal + random_unit_vector(rng);
+        if (near_zero(dir)) dir = normal;
+        next.o = p;
+        next.d = dir;
+        weight = textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo;
+        return true;
+    }
+    case 1u: {  // material.wgsl:22-27: absorbed below the surface
+        const f3 dir = reflect_dir(normalize(ray.d), normal) + m.fuzz * random_in_unit_sphere(rng);
+        next.o = p;
+        next.d = dir;
+        if (dot(dir, normal) > 0.0f) {
+            weight = textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo;
+            return true;
+        }
+        weight = f3{0.0f, 0.0f, 0.0f};  // material_emit of a metal
+        return false;
+    }
+    case 2u: {  // material.wgsl:36-59
+        const float ratio = front ? (1.0f / m.ior) : m.ior;
+        const f3 ud = normalize(ray.d);
+        const float cos_theta = fminf(dot(-ud, normal), 1.0f);
+        const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+        const bool cannot_refract = ratio * sin_theta > 1.0f;
+        bool reflect = cannot_refract;
+        if (!reflect) {  // the draw happens only when refraction is possible
+            float r0 = (1.0f - ratio) / (1.0f + ratio);
+            r0 = r0 * r0;
+            const float refl = r0 + (1.0f - r0) * pow_f(1.0f - cos_theta, 5.0f);
+            reflect = refl > rng_f32(rng);
+        }
+        next.o = p;
+        next.d = reflect ? reflect_dir(ud, normal) : refract_dir(ud, normal, ratio);
+        weight = f3{1.0f, 1.0f, 1.0f};
+        return true;
+    }
+    default:  // diffuse light (3) emits; anything else emits black
+        weight = (m.type == 3u) ? (textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo)
+                                : f3{0.0f, 0.0f, 0.0f};
+        return false;
+    }
+}
+
+}  // namespace orn
