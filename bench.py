@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="also run one plain copy kernel of this size (for scaling profiler "
+                         "traffic counters); reported as calibration_copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path tracer shards by pixel tiles with no "
@@ -105,6 +108,12 @@ def main():
     params.textures_count = len(spec.textures)
     params.current_iteration = 0.0
 
+    calib = None
+    if args.calibration_copy_mib:
+        nbytes = args.calibration_copy_mib << 20
+        ms = dev.calibration_copy(nbytes)
+        calib = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms,
+                 "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
     for _ in range(args.warmup):
         tgt.render(params, spec.spp)
     tgt.wait()
@@ -139,6 +148,8 @@ def main():
                    "compute_units": cus, "bvh_build_s": build_s},
         "kernel_ms_avg": avg_kernel_s * 1e3,
     }
+    if calib:
+        out["calibration_copy"] = calib
     if not args.no_cpu_baseline:
         cb = cpu_baseline(spec, arrays, params, args.cpu_seconds)
         bytes_per_launch = cb["bytes_per_sample"] * spec.samples + 16 * spec.width * spec.height

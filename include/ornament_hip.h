@@ -211,6 +211,19 @@ orn_status orn_render(orn_target *target, const orn_constant_params *params, uin
 orn_status orn_render_async(orn_target *target, const orn_constant_params *params,
                             uint32_t iterations);
 orn_status orn_target_wait(orn_target *target);
+/* Launch tuning of a target's render kernel; any value gives the same image.
+ * regroup_lanes (1..64): a wave interrupts BVH traversal for a shading round
+ *   once this many of its 64 lanes have finished their ray (default 32).
+ * workgroups_per_cu (0..64): persistent 256-lane workgroups launched per
+ *   compute unit; 0 launches as many as can be resident at once. */
+typedef struct orn_tuning {
+    uint32_t regroup_lanes;
+    uint32_t workgroups_per_cu;
+    uint32_t reserved[6];
+} orn_tuning;
+orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
+orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);
+
 /* GPU time of the most recent render's kernels (HIP events on the target's
  * stream), in milliseconds. */
 orn_status orn_target_last_render_ms(orn_target *target, float *ms);
@@ -240,6 +253,12 @@ orn_status orn_trace_rays(orn_device_scene *scene, float t_min, const float *ray
  * op 0 sin(a), 1 cos(a), 2 pow(a, b), 3 acos(a), 4 atan2(a, b). */
 orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, float *out,
                          size_t n);
+
+/* Measurement aid: copies `bytes` (a multiple of 16) from one device buffer to
+ * another with a plain 16-byte-per-lane copy kernel and reports its GPU time.
+ * Profilers use it to scale their traffic counters against a known size, and
+ * it gives the achievable HBM bandwidth of the device. */
+orn_status orn_calibration_copy(orn_ctx *ctx, size_t bytes, float *ms);
 
 #ifdef __cplusplus
 }

@@ -305,6 +305,12 @@ class Device:
               "orn_eval_math")
         return out
 
+    def calibration_copy(self, nbytes):
+        """GPU milliseconds of a plain copy kernel moving `nbytes`."""
+        ms = C.c_float()
+        check(lib().orn_calibration_copy(self._h, int(nbytes), C.byref(ms)), "orn_calibration_copy")
+        return ms.value
+
     def close(self):
         if self._h:
             lib().orn_device_close(self._h)
@@ -376,6 +382,17 @@ class Target:
 
     def wait(self):
         check(lib().orn_target_wait(self._h), "orn_target_wait")
+
+    def set_tuning(self, regroup_lanes=32, workgroups_per_cu=0):
+        t = N.Tuning()
+        t.regroup_lanes = int(regroup_lanes)
+        t.workgroups_per_cu = int(workgroups_per_cu)
+        check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
+
+    def tuning(self):
+        t = N.Tuning()
+        check(lib().orn_target_get_tuning(self._h, C.byref(t)), "orn_target_get_tuning")
+        return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu}
 
     def last_render_ms(self):
         v = C.c_float()

@@ -60,6 +60,10 @@ class SceneDesc(C.Structure):
                 ("textures", C.c_void_p), ("textures_len", C.c_size_t)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("regroup_lanes", u32), ("workgroups_per_cu", u32), ("reserved", u32 * 6)]
+
+
 class Hit(C.Structure):
     _fields_ = [("hit", u32), ("t", f32), ("material_index", u32), ("node_type", u32),
                 ("inverted_transform_id", u32), ("tri_id", u32), ("uv", f32 * 2)]
@@ -92,12 +96,15 @@ DEVICE_API = [
     ("orn_render_async", st, [vp, P(ConstantParams), u32]),
     ("orn_target_wait", st, [vp]),
     ("orn_target_last_render_ms", st, [vp, fp]),
+    ("orn_target_set_tuning", st, [vp, P(Tuning)]),
+    ("orn_target_get_tuning", st, [vp, P(Tuning)]),
     ("orn_get_framebuffer", st, [vp, vp, sz]),
     ("orn_get_accumulation", st, [vp, vp, sz]),
     ("orn_get_rng_state", st, [vp, vp, sz]),
     ("orn_set_state", st, [vp, vp, vp, sz]),
     ("orn_trace_rays", st, [vp, f32, vp, sz, vp]),
     ("orn_eval_math", st, [vp, C.c_int, vp, vp, vp, sz]),
+    ("orn_calibration_copy", st, [vp, sz, fp]),
 ]
 
 HOST_API = [

@@ -41,6 +41,7 @@ struct KernelParams {
     uint32_t tiles_x;
     uint32_t tiles;
     uint32_t tiles_per_xcd;
+    uint32_t regroup;  // waiting lanes that trigger a shading round
 };
 
 __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t per_xcd)
@@ -48,21 +49,40 @@ __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t per_xcd)
     return (b % kXcds) * per_xcd + b / kXcds;
 }
 
-// pathtracer.wgsl:64-79 + 95-161 + 81-93 for one pixel over all iterations
+enum LaneMode : uint32_t { kTraverse = 0u, kWaiting = 1u, kDone = 2u };
+
+// pathtracer.wgsl:64-79 + 95-161 + 81-93, every pixel over all iterations.
+//
+// Persistent lanes: the image is cut into 16x16 tiles, the tiles into one
+// contiguous band per XCD (workgroup b runs on XCD b % 8), and each band is
+// a work queue of pixels. A lane renders one pixel's samples back to back
+// (the per-pixel RNG chain is serial), writes the pixel out and takes the
+// next pixel of its band's queue, so no lane idles while a slow pixel of its
+// wave finishes.
+// Within a wave, a lane alternates between traversing its current ray one
+// node per loop iteration and a shading round (hit shading, accumulation,
+// next camera path, next pixel). A lane whose traversal ended waits; the wave
+// runs a shading round only when at least `regroup` lanes wait (or none is
+// traversing), so node visits run with most lanes busy and the long shading
+// code runs for a batch of lanes at once. Each pixel's own sequence of
+// operations is the reference's, so results do not depend on the schedule.
+template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParams kp,
                                                         float4 *__restrict__ acc_buf,
                                                         uint32_t *__restrict__ rng_buf,
-                                                        float4 *__restrict__ fb)
+                                                        float4 *__restrict__ fb,
+                                                        uint32_t *__restrict__ queues)
 {
     extern __shared__ uint32_t lds_stack[];
-    const uint32_t tile = tile_of_block(blockIdx.x, kp.tiles_per_xcd);
-    if (tile >= kp.tiles) return;
     const orn_constant_params &p = kp.p;
-    const uint32_t x = (tile % kp.tiles_x) * kTile + (threadIdx.x % kTile);
-    const uint32_t y = (tile / kp.tiles_x) * kTile + (threadIdx.x / kTile);
-    if (x >= p.width || y >= p.height) return;
-    const uint32_t pix = y * p.width + x;
     uint32_t *stack = lds_stack + threadIdx.x;
+    const uint32_t band = blockIdx.x % kXcds;
+    const uint32_t band_first_tile = band * kp.tiles_per_xcd;
+    const uint32_t band_tiles =
+        band_first_tile >= kp.tiles ? 0u : min(kp.tiles_per_xcd, kp.tiles - band_first_tile);
+    const uint32_t band_pixels = band_tiles * (kTile * kTile);
+    uint32_t *queue = queues + band;
+    const uint32_t lane = threadIdx.x % 64u;
 
     const orn_camera &cam = p.camera;
     const f3 cam_origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
@@ -71,21 +91,26 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     const f3 cam_v = mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]);
     const f3 cam_u = mk(cam.u[0], cam.u[1], cam.u[2]);
     const f3 cam_vv = mk(cam.v[0], cam.v[1], cam.v[2]);
-    const float inv_w_den = static_cast<float>(p.width - 1u);
-    const float inv_h_den = static_cast<float>(p.height - 1u);
+    const float w_den = static_cast<float>(p.width - 1u);
+    const float h_den = static_cast<float>(p.height - 1u);
+    const uint32_t spp = kp.iterations > 1u ? kp.iterations : 1u;
 
-    uint32_t rng = rng_buf[pix];
-    float4 acc = acc_buf[pix];
-    float cur = p.current_iteration;
-    uint32_t left = kp.iterations > 1u ? kp.iterations : 1u;
-
-    Ray ray;
-    f3 thr;
+    uint32_t x = 0, y = 0, pix = 0;
+    uint32_t rng = 0;
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float cur = 0.0f;
+    uint32_t left = 0;
+    Ray ray;  // world-space ray of the current bounce
+    f3 thr = mk(1.0f, 1.0f, 1.0f);
     uint32_t bounce = 0;
+    Traversal tr;
+    uint32_t mode = kWaiting;
+    bool need_pixel = true;
+
     // camera ray of a new sample: 2 jitter draws + 2 lens draws
     auto start_sample = [&]() {
-        const float u = (static_cast<float>(x) + rng_f32(rng)) / inv_w_den;
-        const float v = (static_cast<float>(y) + rng_f32(rng)) / inv_h_den;
+        const float u = (static_cast<float>(x) + rng_f32(rng)) / w_den;
+        const float v = (static_cast<float>(y) + rng_f32(rng)) / h_den;
         const f3 disk = random_in_unit_disk(rng);
         const f3 rd = cam.lens_radius * disk;
         const f3 offset = rd.x * cam_u + rd.y * cam_vv;
@@ -94,62 +119,121 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         thr = mk(1.0f, 1.0f, 1.0f);
         bounce = 0;
     };
-    start_sample();
+    // pathtracer.wgsl:81-93 and the state write-back of one finished pixel
+    auto finish_pixel = [&]() {
+        acc_buf[pix] = acc;
+        rng_buf[pix] = rng;
+        float4 o;
+        o.x = fminf(fmaxf(acc.x / cur, 0.0f), 1.0f);
+        o.y = fminf(fmaxf(acc.y / cur, 0.0f), 1.0f);
+        o.z = fminf(fmaxf(acc.z / cur, 0.0f), 1.0f);
+        o.w = fminf(fmaxf(acc.w / cur, 0.0f), 1.0f);
+        o.x = pow_f(o.x, p.inverted_gamma);
+        o.y = pow_f(o.y, p.inverted_gamma);
+        o.z = pow_f(o.z, p.inverted_gamma);
+        const uint32_t dst = p.flip_y < 1u ? pix : p.width * (p.height - y - 1u) + x;
+        fb[dst] = o;
+    };
+    // Accumulate finished samples, finish samples that need no traversal
+    // (depth exhausted), and set up the next traversal; a finished pixel
+    // asks for a new one.
+    auto next_traversal = [&](bool done, f3 color) {
+        for (;;) {
+            if (done) {
+                cur = cur + 1.0f;  // State.nextIteration
+                if (cur > 1.0f) {
+                    acc.x = acc.x + color.x;
+                    acc.y = acc.y + color.y;
+                    acc.z = acc.z + color.z;
+                    acc.w = acc.w + 1.0f;
+                } else {
+                    acc = make_float4(color.x, color.y, color.z, 1.0f);
+                }
+                if (--left == 0u) {
+                    finish_pixel();
+                    need_pixel = true;
+                    mode = kWaiting;
+                    return;
+                }
+                start_sample();
+            }
+            if (bounce >= p.depth) {  // depth exhausted: the throughput is returned
+                color = thr;
+                done = true;
+                continue;
+            }
+            trav_begin(sc, ray, tr);
+            mode = kTraverse;
+            return;
+        }
+    };
+
+    const uint32_t regroup = kp.regroup;
     for (;;) {
-        f3 color;
-        bool done = false;
-        if (bounce >= p.depth) {
-            color = thr;  // depth exhausted: the throughput is returned
-            done = true;
-        } else {
-            Hit h;
-            const bool hit = bvh_hit(sc, p.ray_cast_epsilon, ray, stack, kBlock, h);
+        if (mode == kTraverse && !trav_step<F>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
+            mode = kWaiting;
+        const uint64_t waiting = __ballot(mode == kWaiting);
+        const uint64_t traversing = __ballot(mode == kTraverse);
+        if (waiting == 0u && traversing == 0u) break;
+        if (static_cast<uint32_t>(__popcll(waiting)) < regroup && traversing != 0u) continue;
+        // shading round
+        if (mode == kWaiting && !need_pixel) {
             bounce++;
-            if (!hit) {
+            if (!tr.hit) {
                 const f3 ud = normalize(ray.d);
                 const float tt = 0.5f * (ud.y + 1.0f);
                 const float a = (1.0f - tt) * 1.0f;
-                color = thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f);
-                done = true;
+                next_traversal(true, thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f));
             } else {
                 Ray next;
                 f3 w;
-                if (shade(sc, p.textures_count, ray, h, rng, next, w)) {
+                if (shade<F>(sc, p.textures_count, ray, tr.h, rng, next, w)) {
                     ray = next;
                     thr = thr * w;
+                    next_traversal(false, thr);
                 } else {
-                    color = thr * w;
-                    done = true;
+                    next_traversal(true, thr * w);
                 }
             }
         }
-        if (done) {
-            cur = cur + 1.0f;  // State.nextIteration
-            if (cur > 1.0f) {
-                acc.x = acc.x + color.x;
-                acc.y = acc.y + color.y;
-                acc.z = acc.z + color.z;
-                acc.w = acc.w + 1.0f;
-            } else {
-                acc = make_float4(color.x, color.y, color.z, 1.0f);
+        // lanes without a pixel take the next ones of the band's queue, one
+        // atomic per wave; pixels of partial edge tiles are skipped
+        for (;;) {
+            const uint64_t needy = __ballot(mode == kWaiting && need_pixel);
+            if (needy == 0u) break;
+            uint32_t base = 0;
+            if (lane == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(needy)) - 1))
+                base = atomicAdd(queue, static_cast<uint32_t>(__popcll(needy)));
+            base = __shfl(base, __ffsll(static_cast<unsigned long long>(needy)) - 1);
+            if (mode == kWaiting && need_pixel) {
+                const uint32_t q =
+                    base + static_cast<uint32_t>(__popcll(needy & ((1ull << lane) - 1ull)));
+                if (q >= band_pixels) {
+                    mode = kDone;
+                } else {
+                    const uint32_t tile = band_first_tile + q / (kTile * kTile);
+                    const uint32_t within = q % (kTile * kTile);
+                    x = (tile % kp.tiles_x) * kTile + within % kTile;
+                    y = (tile / kp.tiles_x) * kTile + within / kTile;
+                    if (x < p.width && y < p.height) {
+                        pix = y * p.width + x;
+                        rng = rng_buf[pix];
+                        acc = acc_buf[pix];
+                        cur = p.current_iteration;
+                        left = spp;
+                        need_pixel = false;
+                        start_sample();
+                        next_traversal(false, thr);
+                    }
+                }
             }
-            if (--left == 0u) break;
-            start_sample();
         }
     }
-    acc_buf[pix] = acc;
-    rng_buf[pix] = rng;
-    // pathtracer.wgsl:81-93
-    float4 o;
-    o.x = fminf(fmaxf(acc.x / cur, 0.0f), 1.0f);
-    o.y = fminf(fmaxf(acc.y / cur, 0.0f), 1.0f);
-    o.z = fminf(fmaxf(acc.z / cur, 0.0f), 1.0f);
-    o.w = fminf(fmaxf(acc.w / cur, 0.0f), 1.0f);
-    o.x = pow_f(o.x, p.inverted_gamma);
-    o.y = pow_f(o.y, p.inverted_gamma);
-    o.z = pow_f(o.z, p.inverted_gamma);
-    const uint32_t dst = p.flip_y < 1u ? pix : p.width * (p.height - y - 1u) + x;
-    fb[dst] = o;
+}
+
+__global__ void reset_queues_kernel(uint32_t *q)
+{
+    if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
@@ -163,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
     r.o = mk(rays[6u * i], rays[6u * i + 1u], rays[6u * i + 2u]);
     r.d = mk(rays[6u * i + 3u], rays[6u * i + 4u], rays[6u * i + 5u]);
     Hit h{};
-    const bool hit = bvh_hit(sc, t_min, r, lds_stack + threadIdx.x, kBlock, h);
+    const bool hit = bvh_hit<kHasSpheres | kHasMeshes>(sc, t_min, r, lds_stack + threadIdx.x, kBlock, h);
     orn_hit o{};
     o.hit = hit ? 1u : 0u;
     if (hit) {
@@ -191,6 +275,12 @@ __global__ void math_kernel(int op, const float *a, const float *b, float *out, 
     default: r = atan2_f(a[i], b[i]); break;
     }
     out[i] = r;
+}
+
+__global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, uint64_t n)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 __global__ void seed_rng_kernel(uint32_t *rng, uint32_t n, uint32_t base)
@@ -230,15 +320,19 @@ struct orn_device_scene {
     DevScene ds;
     std::vector<void *> allocations;
     uint32_t stack_slots;
+    uint32_t features;  // SceneFeatures present in the arrays
     int refs;
 };
 
 struct orn_target {
     orn_device_scene *scene;
+    orn_tuning tuning;
     uint32_t width, height;
     float4 *fb;
     float4 *acc;
     uint32_t *rng;
+    uint32_t *queues;  // one pixel-queue counter per XCD band
+    uint32_t grid;     // persistent workgroups of the render kernel
     hipEvent_t ev_start, ev_stop;
     bool timed;
 };
@@ -377,6 +471,53 @@ orn_status validate(const orn_scene_desc *d, uint32_t *stack_slots)
 
 uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
+orn_tuning default_tuning()
+{
+    orn_tuning t{};
+    t.regroup_lanes = 32;
+    t.workgroups_per_cu = 0;  // as many as can be resident at once
+    return t;
+}
+
+// One instantiation per combination of scene features.
+// Persistent grid: workgroups_per_cu per compute unit (0: as many as can be
+// resident at once), a multiple of the XCD count, at most one per tile.
+template <uint32_t F>
+void launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
+                     const KernelParams &kp, float4 *acc, uint32_t *rng, float4 *fb, uint32_t *queues)
+{
+    if (per_cu == 0) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F>, kBlock, lds) != hipSuccess ||
+            blocks < 1)
+            blocks = 1;
+        per_cu = static_cast<uint32_t>(blocks);
+    }
+    uint32_t grid = cus * per_cu;
+    grid = std::min(grid, kp.tiles_per_xcd * kXcds);
+    grid = std::max(kXcds, grid / kXcds * kXcds);
+    render_kernel<F><<<grid, kBlock, lds, st>>>(ds, kp, acc, rng, fb, queues);
+}
+
+void launch_render(uint32_t features, uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st,
+                   const DevScene &ds, const KernelParams &kp, float4 *acc, uint32_t *rng, float4 *fb,
+                   uint32_t *queues)
+{
+#define ORN_CASE(F) \
+    case F: launch_render_f<F>(per_cu, cus, lds, st, ds, kp, acc, rng, fb, queues); break;
+    switch (features & 7u) {
+        ORN_CASE(0u)
+        ORN_CASE(1u)
+        ORN_CASE(2u)
+        ORN_CASE(3u)
+        ORN_CASE(4u)
+        ORN_CASE(5u)
+        ORN_CASE(6u)
+        ORN_CASE(7u)
+    }
+#undef ORN_CASE
+}
+
 }  // namespace
 
 extern "C" {
@@ -472,6 +613,11 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     ctx->refs++;
     s->refs = 1;
     s->stack_slots = slots;
+    s->features = desc->textures_len ? kHasTextures : 0u;
+    for (size_t i = 0; i < desc->tlas_nodes_len; i++) {
+        if (desc->tlas_nodes[i].node_type == 1u) s->features |= kHasSpheres;
+        if (desc->tlas_nodes[i].node_type == 2u) s->features |= kHasMeshes;
+    }
     DevScene &ds = s->ds;
     auto fail = [&](orn_status e) {
         release_scene(s);
@@ -566,6 +712,7 @@ orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t heigh
     if (!t) return ORN_ERROR_OUT_OF_MEMORY;
     t->scene = s;
     s->refs++;
+    t->tuning = default_tuning();
     t->width = width;
     t->height = height;
     auto fail = [&](hipError_t e) {
@@ -576,6 +723,7 @@ orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t heigh
     if ((e = hipMalloc(&t->fb, n * sizeof(float4))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&t->acc, n * sizeof(float4))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&t->rng, n * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&t->queues, 64 * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipEventCreate(&t->ev_start)) != hipSuccess) return fail(e);
     if ((e = hipEventCreate(&t->ev_stop)) != hipSuccess) return fail(e);
     hipStream_t st = s->ctx->stream;
@@ -596,6 +744,7 @@ orn_status orn_target_destroy(orn_target *t)
     if (t->fb) (void)hipFree(t->fb);
     if (t->acc) (void)hipFree(t->acc);
     if (t->rng) (void)hipFree(t->rng);
+    if (t->queues) (void)hipFree(t->queues);
     if (t->ev_start) (void)hipEventDestroy(t->ev_start);
     if (t->ev_stop) (void)hipEventDestroy(t->ev_stop);
     orn_device_scene *s = t->scene;
@@ -617,11 +766,13 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     kp.tiles_x = blocks_for(t->width, kTile);
     kp.tiles = kp.tiles_x * blocks_for(t->height, kTile);
     kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
-    const uint32_t grid = kp.tiles_per_xcd * kXcds;
+    kp.regroup = t->tuning.regroup_lanes;
     const size_t lds = size_t(s->stack_slots) * kBlock * sizeof(uint32_t);
     hipStream_t st = s->ctx->stream;
     ORN_TRY(hipEventRecord(t->ev_start, st));
-    render_kernel<<<grid, kBlock, lds, st>>>(s->ds, kp, t->acc, t->rng, t->fb);
+    reset_queues_kernel<<<1, 64, 0, st>>>(t->queues);
+    launch_render(s->features, t->tuning.workgroups_per_cu, static_cast<uint32_t>(s->ctx->compute_units), lds,
+                  st, s->ds, kp, t->acc, t->rng, t->fb, t->queues);
     ORN_TRY(hipGetLastError());
     ORN_TRY(hipEventRecord(t->ev_stop, st));
     t->timed = true;
@@ -641,6 +792,23 @@ orn_status orn_render(orn_target *t, const orn_constant_params *params, uint32_t
     orn_status s = orn_render_async(t, params, iterations);
     if (s != ORN_OK) return s;
     return orn_target_wait(t);
+}
+
+orn_status orn_target_set_tuning(orn_target *t, const orn_tuning *tuning)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64)
+        return ORN_ERROR_INVALID_VALUE;
+    t->tuning = *tuning;
+    return ORN_OK;
+}
+
+orn_status orn_target_get_tuning(orn_target *t, orn_tuning *tuning)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    if (!tuning) return ORN_ERROR_INVALID_VALUE;
+    *tuning = t->tuning;
+    return ORN_OK;
 }
 
 orn_status orn_target_last_render_ms(orn_target *t, float *ms)
@@ -743,6 +911,39 @@ orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, f
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipStreamSynchronize(st);
     if (d) (void)hipFree(d);
+    return from_hip(e);
+}
+
+orn_status orn_calibration_copy(orn_ctx *ctx, size_t bytes, float *ms)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    if (!ms || bytes == 0 || bytes % 16) return ORN_ERROR_INVALID_VALUE;
+    ORN_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const uint64_t n = bytes / 16;
+    if (e == hipSuccess) {
+        zero_kernel<<<blocks_for(static_cast<uint32_t>(std::min<uint64_t>(n, 1u << 30)), 256), 256, 0, st>>>(
+            a, b, static_cast<uint32_t>(std::min<uint64_t>(n, 1u << 30)));
+        e = hipEventRecord(e0, st);
+    }
+    if (e == hipSuccess) {
+        copy_kernel<<<ctx->compute_units * 8, 256, 0, st>>>(a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
     return from_hip(e);
 }
 

@@ -152,84 +152,132 @@ __device__ __forceinline__ f3 xform_normal(const float4 *m, f3 n)
               c2.x * n.x + c2.y * n.y + c2.z * n.z + c2.w * 0.0f};
 }
 
-// bvh.wgsl:14-137. `stack` is this lane's LDS column, `stride` the block size.
+// Scene content known at launch: kernels are instantiated without the code
+// (and registers) of primitive kinds a scene does not contain.
+enum SceneFeatures : uint32_t {
+    kHasSpheres = 1u,
+    kHasMeshes = 2u,
+    kHasTextures = 4u,
+};
+
+// bvh.wgsl:14-137 as a resumable state machine: trav_begin() sets up one ray,
+// trav_step() visits one node and returns false once the ray is finished.
 // Same visiting order as the reference (left pushed before right, so the
-// right child is popped first) and the same strict `t < t_max` update, so
-// ties between equal distances resolve the same way.
+// right child is taken first) and the same strict `t < t_max` update, so
+// ties between equal distances resolve the same way. The node taken next is
+// kept in a register: only nodes the reference leaves on its stack go to the
+// lane's LDS column (`stack`, stride = block size).
+struct Traversal {
+    Ray ray;        // current space (world, or object space inside a BLAS)
+    f3 inv, ox;     // safe_invdir(ray.d) and -ray.o * inv
+    float t_max;
+    uint32_t addr;  // next node to visit
+    uint32_t top;   // pending entries in the LDS column
+    uint32_t material_index, inv_tid;  // of the mesh being traversed
+    bool in_tlas;
+    bool hit;
+    Hit h;
+};
+
+__device__ __forceinline__ void trav_begin(const DevScene &sc, const Ray &world, Traversal &tr)
+{
+    tr.ray = world;
+    tr.inv = safe_invdir(world.d);
+    tr.ox = (-world.o) * tr.inv;
+    tr.t_max = kTMaxInit;
+    tr.addr = sc.tlas_len - 1u;
+    tr.top = 0u;
+    tr.material_index = 0u;
+    tr.inv_tid = 0u;
+    tr.in_tlas = true;
+    tr.hit = false;
+}
+
+template <uint32_t F>
+__device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const Ray &world,
+                                          uint32_t *stack, uint32_t stride, Traversal &tr)
+{
+    const float4 *node =
+        ((F & kHasMeshes) ? (tr.in_tlas ? sc.tlas : sc.blas) : sc.tlas) + 4u * tr.addr;
+    const float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
+    const uint32_t type = as_u32(n2.w);
+    const uint32_t cap = sc.stack_slots;
+    if (type == 0u) {
+        const bool l = aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z),
+                                tr.inv, tr.ox, t_min, tr.t_max);
+        const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z),
+                                tr.inv, tr.ox, t_min, tr.t_max);
+        if (r) {
+            if (l && tr.top < cap) stack[(tr.top++) * stride] = as_u32(n0.w);
+            tr.addr = as_u32(n1.w);
+            return true;
+        }
+        if (l) {
+            tr.addr = as_u32(n0.w);
+            return true;
+        }
+    } else if ((F & kHasSpheres) && type == 1u) {
+        const uint32_t tid = as_u32(n3.w) * 2u;
+        const float t = sphere_hit(xform_ray(sc.transforms + 4u * tid, tr.ray), t_min, tr.t_max);
+        if (t < tr.t_max) {
+            tr.hit = true;
+            tr.t_max = t;
+            tr.h.t = t;
+            tr.h.material = as_u32(n1.w);
+            tr.h.node_type = 1u;
+            tr.h.inv_tid = tid;
+        }
+    } else if ((F & kHasMeshes) && type == 2u) {
+        tr.in_tlas = false;
+        if (tr.top < cap) stack[(tr.top++) * stride] = kFinishedTraverseBlas;
+        tr.inv_tid = as_u32(n3.w) * 2u;
+        tr.material_index = as_u32(n1.w);
+        tr.ray = xform_ray(sc.transforms + 4u * tr.inv_tid, tr.ray);
+        tr.inv = safe_invdir(tr.ray.d);
+        tr.ox = (-tr.ray.o) * tr.inv;
+        tr.addr = as_u32(n0.w);
+        return true;
+    } else if ((F & kHasMeshes) && type == 3u) {
+        float u = 0.0f, v = 0.0f;
+        const float t = triangle_hit(tr.ray, xyz(n0), xyz(n1), xyz(n2), t_min, tr.t_max, u, v);
+        if (t < tr.t_max) {
+            tr.hit = true;
+            tr.t_max = t;
+            tr.h.t = t;
+            tr.h.material = tr.material_index;
+            tr.h.node_type = 2u;
+            tr.h.inv_tid = tr.inv_tid;
+            tr.h.tri_id = as_u32(n0.w) * 3u;
+            tr.h.u = u;
+            tr.h.v = v;
+        }
+    }
+    if (tr.top == 0u) return false;
+    tr.addr = stack[(--tr.top) * stride];
+    if ((F & kHasMeshes) && tr.addr == kFinishedTraverseBlas) {
+        // back in the TLAS: the world-space ray and its reciprocal are
+        // recomputed (the same operations give the same values)
+        tr.in_tlas = true;
+        tr.ray = world;
+        tr.inv = safe_invdir(world.d);
+        tr.ox = (-world.o) * tr.inv;
+        if (tr.top == 0u) return false;
+        tr.addr = stack[(--tr.top) * stride];
+    }
+    return true;
+}
+
+// The whole of bvh_hit for one ray.
+template <uint32_t F>
 __device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const Ray &world,
                                         uint32_t *stack, uint32_t stride, Hit &h)
 {
-    float t_max = kTMaxInit;
-    uint32_t top = 0;  // entries above the reference's bottom slot
-    uint32_t addr = sc.tlas_len - 1u;
-    bool in_tlas = true;
-    bool hit_anything = false;
-    Ray ray = world;
-    f3 inv = safe_invdir(ray.d);
-    f3 ox = (-ray.o) * inv;
-    const f3 inv_w = inv, ox_w = ox;
-    uint32_t material_index = 0u, inv_tid = 0u;
-    const uint32_t cap = sc.stack_slots;
-    for (;;) {
-        const float4 *node = (in_tlas ? sc.tlas : sc.blas) + 4u * addr;
-        const float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
-        const uint32_t type = as_u32(n2.w);
-        if (type == 0u) {
-            const bool l = aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z), inv,
-                                    ox, t_min, t_max);
-            const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z), inv,
-                                    ox, t_min, t_max);
-            if (l && top < cap) stack[(top++) * stride] = as_u32(n0.w);
-            if (r && top < cap) stack[(top++) * stride] = as_u32(n1.w);
-        } else if (type == 1u) {
-            const uint32_t tid = as_u32(n3.w) * 2u;
-            const float t = sphere_hit(xform_ray(sc.transforms + 4u * tid, ray), t_min, t_max);
-            if (t < t_max) {
-                hit_anything = true;
-                t_max = t;
-                h.t = t;
-                h.material = as_u32(n1.w);
-                h.node_type = 1u;
-                h.inv_tid = tid;
-            }
-        } else if (type == 2u) {
-            in_tlas = false;
-            if (top + 2u <= cap) {
-                stack[(top++) * stride] = kFinishedTraverseBlas;
-                stack[(top++) * stride] = as_u32(n0.w);
-            }
-            inv_tid = as_u32(n3.w) * 2u;
-            material_index = as_u32(n1.w);
-            ray = xform_ray(sc.transforms + 4u * inv_tid, ray);
-            inv = safe_invdir(ray.d);
-            ox = (-ray.o) * inv;
-        } else if (type == 3u) {
-            float u = 0.0f, v = 0.0f;
-            const float t = triangle_hit(ray, xyz(n0), xyz(n1), xyz(n2), t_min, t_max, u, v);
-            if (t < t_max) {
-                hit_anything = true;
-                t_max = t;
-                h.t = t;
-                h.material = material_index;
-                h.node_type = 2u;
-                h.inv_tid = inv_tid;
-                h.tri_id = as_u32(n0.w) * 3u;
-                h.u = u;
-                h.v = v;
-            }
-        }
-        if (top == 0u) break;
-        addr = stack[(--top) * stride];
-        if (addr == kFinishedTraverseBlas) {
-            in_tlas = true;
-            ray = world;
-            inv = inv_w;
-            ox = ox_w;
-            if (top == 0u) break;
-            addr = stack[(--top) * stride];
-        }
+    Traversal tr;
+    trav_begin(sc, world, tr);
+    while (trav_step<F>(sc, t_min, world, stack, stride, tr)) {
     }
-    return hit_anything;
+    h = tr.h;
+    return tr.hit;
 }
 
 // ---------------------------------------------------------------------------
@@ -334,17 +382,19 @@ __device__ __forceinline__ f3 refract_dir(f3 e1, f3 e2, float eta)
 // Shading of one closest hit: fills the next ray and the attenuation, or the
 // emitted radiance when the path ends (pathtracer.wgsl:114-157).
 // Returns true when the path continues.
+template <uint32_t F>
 __device__ __forceinline__ bool shade(const DevScene &sc, uint32_t textures_count, const Ray &ray,
                                       const Hit &h, uint32_t &rng, Ray &next, f3 &weight)
 {
     const f3 p = ray.o + h.t * ray.d;
     const Material m = load_material(sc, h.material);
-    const bool textured = m.texture < textures_count && m.texture < sc.textures_len;
+    const bool textured =
+        (F & kHasTextures) && m.texture < textures_count && m.texture < sc.textures_len;
     f3 normal;
     bool front;
     float tu = 0.0f, tv = 0.0f;
     f3 outward;
-    if (h.node_type == 1u) {
+    if ((F & kHasSpheres) && (!(F & kHasMeshes) || h.node_type == 1u)) {
         // sphere: outward = normalize(p - M * (0,0,0,1))
         const float4 c3 = sc.transforms[4u * (h.inv_tid + 1u) + 3u];
         const float4 c0 = sc.transforms[4u * (h.inv_tid + 1u)];
@@ -384,26 +434,31 @@ __device__ __forceinline__ bool shade(const DevScene &sc, uint32_t textures_coun
         normal = outward;
         front = true;
     }
-    switch (m.type) {
-    case 0u: {  // material.wgsl:9-20
-        f3 dir = normal + random_unit_vector(rng);
-        if (near_zero(dir)) dir = normal;
+    if (m.type <= 1u) {
+        // Lambertian (material.wgsl:9-20) and metal (:22-27) draw the same
+        // three numbers for random_in_unit_sphere; one code path serves both
+        // kinds of lanes in a wave.
+        const f3 s = random_in_unit_sphere(rng);
+        f3 dir;
+        bool cont;
+        if (m.type == 0u) {
+            dir = normal + normalize(s);
+            if (near_zero(dir)) dir = normal;
+            cont = true;
+        } else {
+            dir = reflect_dir(normalize(ray.d), normal) + m.fuzz * s;
+            cont = dot(dir, normal) > 0.0f;  // absorbed below the surface
+        }
         next.o = p;
         next.d = dir;
-        weight = textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo;
-        return true;
-    }
-    case 1u: {  // material.wgsl:22-27: absorbed below the surface
-        const f3 dir = reflect_dir(normalize(ray.d), normal) + m.fuzz * random_in_unit_sphere(rng);
-        next.o = p;
-        next.d = dir;
-        if (dot(dir, normal) > 0.0f) {
+        if (cont) {
             weight = textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo;
             return true;
         }
         weight = f3{0.0f, 0.0f, 0.0f};  // material_emit of a metal
         return false;
     }
+    switch (m.type) {
     case 2u: {  // material.wgsl:36-59
         const float ratio = front ? (1.0f / m.ior) : m.ior;
         const f3 ud = normalize(ray.d);
