@@ -25,6 +25,8 @@ def check_subset(prod, spec, acc_g, rng_g, pixels):
 
 @pytest.mark.parametrize("name", ["C2", "C3"])
 def test_full_size_render(device, name):
+    """BVH4 (default) render at full size; the spp/2 + spp/2 render uses the
+    BVH2 and a different schedule, and must give the same bits."""
     spec = W.random_spheres() if name == "C2" else W.instanced_icospheres()
     print(f"[full] {name} {spec.width}x{spec.height} {spec.spp} spp depth {spec.depth}", flush=True)
     prod = zo.Bvh(W.build_scene(spec))
@@ -43,6 +45,7 @@ def test_full_size_render(device, name):
     assert bad.mean() < 1e-5
     # determinism and progressive equivalence: spp/2 + spp/2 == spp
     b = zo.Target(ds, spec.width, spec.height)
+    b.set_tuning(regroup_lanes=16, bvh_width=2)
     b.render(zo_params(spec), spec.spp // 2)
     b.render(zo_params(spec, current_iteration=spec.spp // 2), spec.spp // 2)
     assert same_bits(b.accumulation(), acc_a)

@@ -27,13 +27,16 @@ def small_specs():
     }
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
-def test_trace_rays_match_oracle(device, name):
+def test_trace_rays_match_oracle(device, name, width):
+    """Both node widths: the reference's BVH2 and the derived BVH4."""
     spec = small_specs()[name]
     prod, orc = both_arrays(spec)
     rays = ray_set(spec, prod, 20000, seed=7)
-    print(f"[trace] {name}: {len(rays)} rays", flush=True)
+    print(f"[trace] {name} width {width}: {len(rays)} rays", flush=True)
     ds = zo.DeviceScene(device, prod)
+    ds.set_trace_width(width)
     g = ds.trace(rays, spec.ray_cast_epsilon)
     o = O.OracleScene(orc).trace(rays, spec.ray_cast_epsilon)
     assert np.array_equal(g["hit"], o["hit"])
@@ -76,13 +79,21 @@ def test_trace_rays_match_fp64_brute_force(device, name):
     assert np.all(rel <= np.maximum(1e-5, bound))
 
 
+@pytest.mark.parametrize("tuning", [dict(bvh_width=4), dict(bvh_width=2),
+                                    dict(bvh_width=4, regroup_lanes=1, workgroups_per_cu=1),
+                                    dict(bvh_width=2, regroup_lanes=64, workgroups_per_cu=3)])
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
-def test_image_matches_oracle(device, name):
+def test_image_matches_oracle(device, name, tuning):
+    """Any launch tuning (node width, regrouping, persistent grid size)
+    renders the oracle's image bit for bit."""
     spec = small_specs()[name]
     prod, _ = both_arrays(spec)
     n = spec.width * spec.height
-    print(f"[image] {name}: {spec.width}x{spec.height} {spec.spp} spp", flush=True)
-    ds, tgt = device_render(device, prod, spec)
+    print(f"[image] {name}: {spec.width}x{spec.height} {spec.spp} spp, {tuning}", flush=True)
+    ds = zo.DeviceScene(device, prod)
+    tgt = zo.Target(ds, spec.width, spec.height)
+    tgt.set_tuning(**tuning)
+    tgt.render(zo_params(spec), spec.spp)
     acc_g, fb_g, rng_g = tgt.accumulation(), tgt.framebuffer(), tgt.rng_state()
     p = to_oracle_params(zo_params(spec))
     acc_o = np.zeros((n, 4), np.float32)

@@ -342,6 +342,9 @@ class DeviceScene:
         check(lib().orn_scene_stack_depth(self._h, C.byref(v)), "orn_scene_stack_depth")
         return v.value
 
+    def set_trace_width(self, width):
+        check(lib().orn_scene_set_trace_width(self._h, int(width)), "orn_scene_set_trace_width")
+
     def trace(self, rays, t_min=0.001):
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
         out = np.zeros(len(r), dtype=HIT_DTYPE)
@@ -383,16 +386,18 @@ class Target:
     def wait(self):
         check(lib().orn_target_wait(self._h), "orn_target_wait")
 
-    def set_tuning(self, regroup_lanes=32, workgroups_per_cu=0):
+    def set_tuning(self, regroup_lanes=32, workgroups_per_cu=0, bvh_width=4):
         t = N.Tuning()
         t.regroup_lanes = int(regroup_lanes)
         t.workgroups_per_cu = int(workgroups_per_cu)
+        t.bvh_width = int(bvh_width)
         check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
 
     def tuning(self):
         t = N.Tuning()
         check(lib().orn_target_get_tuning(self._h, C.byref(t)), "orn_target_get_tuning")
-        return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu}
+        return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
+                "bvh_width": t.bvh_width}
 
     def last_render_ms(self):
         v = C.c_float()

@@ -61,7 +61,8 @@ class SceneDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("regroup_lanes", u32), ("workgroups_per_cu", u32), ("reserved", u32 * 6)]
+    _fields_ = [("regroup_lanes", u32), ("workgroups_per_cu", u32), ("bvh_width", u32),
+                ("reserved", u32 * 5)]
 
 
 class Hit(C.Structure):
@@ -90,6 +91,7 @@ DEVICE_API = [
     ("orn_scene_destroy", st, [vp]),
     ("orn_scene_validate", st, [P(SceneDesc), up]),
     ("orn_scene_stack_depth", st, [vp, up]),
+    ("orn_scene_set_trace_width", st, [vp, u32]),
     ("orn_target_create", st, [vp, u32, u32, u32, P(vp)]),
     ("orn_target_destroy", st, [vp]),
     ("orn_render", st, [vp, P(ConstantParams), u32]),

@@ -66,7 +66,7 @@ enum LaneMode : uint32_t { kTraverse = 0u, kWaiting = 1u, kDone = 2u };
 // traversing), so node visits run with most lanes busy and the long shading
 // code runs for a batch of lanes at once. Each pixel's own sequence of
 // operations is the reference's, so results do not depend on the schedule.
-template <uint32_t F>
+template <uint32_t F, uint32_t W>
 __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParams kp,
                                                         float4 *__restrict__ acc_buf,
                                                         uint32_t *__restrict__ rng_buf,
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                 done = true;
                 continue;
             }
-            trav_begin(sc, ray, tr);
+            trav_begin<W>(sc, ray, tr);
             mode = kTraverse;
             return;
         }
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
 
     const uint32_t regroup = kp.regroup;
     for (;;) {
-        if (mode == kTraverse && !trav_step<F>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
+        if (mode == kTraverse && !trav_step<F, W>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
             mode = kWaiting;
         const uint64_t waiting = __ballot(mode == kWaiting);
         const uint64_t traversing = __ballot(mode == kTraverse);
@@ -236,6 +236,7 @@ __global__ void reset_queues_kernel(uint32_t *q)
     if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
 }
 
+template <uint32_t W>
 __global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
                                                        const float *__restrict__ rays, uint32_t n,
                                                        orn_hit *__restrict__ out)
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
     r.o = mk(rays[6u * i], rays[6u * i + 1u], rays[6u * i + 2u]);
     r.d = mk(rays[6u * i + 3u], rays[6u * i + 4u], rays[6u * i + 5u]);
     Hit h{};
-    const bool hit = bvh_hit<kHasSpheres | kHasMeshes>(sc, t_min, r, lds_stack + threadIdx.x, kBlock, h);
+    const bool hit = bvh_hit<kHasSpheres | kHasMeshes, W>(sc, t_min, r, lds_stack + threadIdx.x, kBlock, h);
     orn_hit o{};
     o.hit = hit ? 1u : 0u;
     if (hit) {
@@ -319,8 +320,10 @@ struct orn_device_scene {
     orn_ctx *ctx;
     DevScene ds;
     std::vector<void *> allocations;
-    uint32_t stack_slots;
-    uint32_t features;  // SceneFeatures present in the arrays
+    uint32_t stack_slots;   // BVH2 traversal
+    uint32_t stack_slots4;  // BVH4 traversal
+    uint32_t trace_width;   // node width orn_trace_rays uses
+    uint32_t features;      // SceneFeatures present in the arrays
     int refs;
 };
 
@@ -401,10 +404,108 @@ std::vector<uint32_t> node_depths(const orn_bvh_node *nodes, size_t n)
     return d;
 }
 
+// Derived BVH4: every BVH2 internal node N becomes a BVH4 node whose slots
+// hold N's grandchildren [LL, LR, RL, RR] (a child that is a leaf stands for
+// itself), with the boxes the BVH2 stores for them. Leaves stay in the BVH2
+// arrays and are referenced as kLeafRef | index.
+struct Bvh4Node {
+    float lo[3][4];  // [axis][slot]
+    float hi[3][4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "BVH4 node is 128 bytes");
+
+struct Bvh4Builder {
+    const orn_bvh_node *nodes;
+    std::vector<Bvh4Node> out;
+    std::vector<uint32_t> memo;  // BVH2 index -> reference, 0xffffffff = not built
+
+    explicit Bvh4Builder(const orn_bvh_node *n, size_t len) : nodes(n), memo(len, 0xffffffffu) {}
+
+    uint32_t build(uint32_t idx)
+    {
+        if (nodes[idx].node_type != 0u) return kLeafRef | idx;
+        if (memo[idx] != 0xffffffffu) return memo[idx];
+        struct Entry {
+            uint32_t idx;
+            const float *lo, *hi;
+        } e[4];
+        int n = 0;
+        const orn_bvh_node &nd = nodes[idx];
+        const uint32_t kids[2] = {nd.left_or_custom_id, nd.right_or_material_index};
+        const float *klo[2] = {nd.lmin_or_v0, nd.rmin_or_v2}, *khi[2] = {nd.lmax_or_v1, nd.rmax_or_v3};
+        for (int k = 0; k < 2; k++) {
+            const orn_bvh_node &c = nodes[kids[k]];
+            if (c.node_type == 0u) {
+                e[n++] = {c.left_or_custom_id, c.lmin_or_v0, c.lmax_or_v1};
+                e[n++] = {c.right_or_material_index, c.rmin_or_v2, c.rmax_or_v3};
+            } else {
+                e[n++] = {kids[k], klo[k], khi[k]};
+            }
+        }
+        const uint32_t me = static_cast<uint32_t>(out.size());
+        out.emplace_back();
+        memo[idx] = me;
+        Bvh4Node node{};
+        for (int i = 0; i < 4; i++) {
+            for (int a = 0; a < 3; a++) {
+                node.lo[a][i] = i < n ? e[i].lo[a] : 0.0f;
+                node.hi[a][i] = i < n ? e[i].hi[a] : 0.0f;
+            }
+            node.child[i] = kEmptyRef;
+        }
+        for (int i = 0; i < n; i++) node.child[i] = build(e[i].idx);
+        out[me] = node;
+        return me;
+    }
+};
+
+// Worst-case pending stack entries of the traversal from a node on, for
+// BVH2 (take the right child, push the left) and BVH4 (push all hit slots
+// but the last). A mesh leaf of the TLAS pushes the BLAS marker and
+// continues into its BLAS.
+uint32_t pending2(const orn_bvh_node *n, uint32_t i, const orn_bvh_node *blas, std::vector<uint32_t> &memo_blas,
+                  bool tlas)
+{
+    const orn_bvh_node &nd = n[i];
+    if (nd.node_type == 0u) {
+        const uint32_t l = pending2(n, nd.left_or_custom_id, blas, memo_blas, tlas);
+        const uint32_t r = pending2(n, nd.right_or_material_index, blas, memo_blas, tlas);
+        return std::max(1u + r, l);
+    }
+    if (tlas && nd.node_type == 2u) {
+        uint32_t &m = memo_blas[nd.left_or_custom_id];
+        if (m == 0xffffffffu) m = pending2(blas, nd.left_or_custom_id, blas, memo_blas, false);
+        return 1u + m;
+    }
+    return 0u;
+}
+
+uint32_t pending4(const std::vector<Bvh4Node> &n4, uint32_t ref, const orn_bvh_node *leaves,
+                  const std::vector<Bvh4Node> *blas4, std::vector<uint32_t> *memo_blas, const uint32_t *blas_ref)
+{
+    if (ref & kLeafRef) {
+        const orn_bvh_node &nd = leaves[ref & ~kLeafRef];
+        if (blas4 && nd.node_type == 2u) {
+            uint32_t &m = (*memo_blas)[nd.left_or_custom_id];
+            if (m == 0xffffffffu) m = pending4(*blas4, blas_ref[nd.left_or_custom_id], nullptr, nullptr, nullptr, nullptr);
+            return 1u + m;
+        }
+        return 0u;
+    }
+    const Bvh4Node &nd = n4[ref];
+    uint32_t worst = 0;
+    for (uint32_t i = 0; i < 4; i++)
+        if (nd.child[i] != kEmptyRef)
+            worst = std::max(worst, i + pending4(n4, nd.child[i], leaves, blas4, memo_blas, blas_ref));
+    return worst;
+}
+
 orn_status validate(const orn_scene_desc *d, uint32_t *stack_slots)
 {
     if (!d->tlas_nodes || d->tlas_nodes_len == 0) return ORN_ERROR_EMPTY_SCENE;
-    if (d->tlas_nodes_len > 0x7fffffffu || d->blas_nodes_len > 0x7fffffffu) return ORN_ERROR_INVALID_SCENE;
+    if (d->tlas_nodes_len >= 0x7ffffff0u || d->blas_nodes_len >= 0x7ffffff0u) return ORN_ERROR_INVALID_SCENE;
     auto need = [](const void *p, size_t n) { return n == 0 || p != nullptr; };
     if (!need(d->blas_nodes, d->blas_nodes_len) || !need(d->normals, d->normals_len) ||
         !need(d->normal_indices, d->normal_indices_len) || !need(d->uvs, d->uvs_len) ||
@@ -456,14 +557,17 @@ orn_status validate(const orn_scene_desc *d, uint32_t *stack_slots)
         if (!comps_ok || !fmt_ok || t.width == 0 || t.height == 0) return ORN_ERROR_UNSUPPORTED_TEXTURE;
         if (!t.data) return ORN_ERROR_INVALID_VALUE;
     }
+    // trees deeper than the reference's 64-entry stack are rejected before
+    // anything recurses over them
     const std::vector<uint32_t> td = node_depths(d->tlas_nodes, d->tlas_nodes_len);
     const std::vector<uint32_t> bd = node_depths(d->blas_nodes, d->blas_nodes_len);
     uint32_t blas_depth = 0;
     for (size_t i = 0; i < d->blas_nodes_len; i++)
         if (blas_root_used[i]) blas_depth = std::max(blas_depth, bd[i]);
-    // pending entries: one per TLAS level + 2 children, the BLAS marker and
-    // root, one per BLAS level + 2 children
-    const uint32_t slots = td[d->tlas_nodes_len - 1] + blas_depth + 4u;
+    if (td[d->tlas_nodes_len - 1] + blas_depth + 2u > kMaxStack) return ORN_ERROR_BVH_TOO_DEEP;
+    std::vector<uint32_t> memo(d->blas_nodes_len, 0xffffffffu);
+    const uint32_t slots = std::max(1u, pending2(d->tlas_nodes, static_cast<uint32_t>(d->tlas_nodes_len - 1),
+                                                 d->blas_nodes, memo, true));
     if (slots > kMaxStack) return ORN_ERROR_BVH_TOO_DEEP;
     *stack_slots = slots;
     return ORN_OK;
@@ -476,19 +580,20 @@ orn_tuning default_tuning()
     orn_tuning t{};
     t.regroup_lanes = 32;
     t.workgroups_per_cu = 0;  // as many as can be resident at once
+    t.bvh_width = 2;
     return t;
 }
 
 // One instantiation per combination of scene features.
 // Persistent grid: workgroups_per_cu per compute unit (0: as many as can be
 // resident at once), a multiple of the XCD count, at most one per tile.
-template <uint32_t F>
+template <uint32_t F, uint32_t W>
 void launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
                      const KernelParams &kp, float4 *acc, uint32_t *rng, float4 *fb, uint32_t *queues)
 {
     if (per_cu == 0) {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F>, kBlock, lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F, W>, kBlock, lds) != hipSuccess ||
             blocks < 1)
             blocks = 1;
         per_cu = static_cast<uint32_t>(blocks);
@@ -496,15 +601,20 @@ void launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, 
     uint32_t grid = cus * per_cu;
     grid = std::min(grid, kp.tiles_per_xcd * kXcds);
     grid = std::max(kXcds, grid / kXcds * kXcds);
-    render_kernel<F><<<grid, kBlock, lds, st>>>(ds, kp, acc, rng, fb, queues);
+    render_kernel<F, W><<<grid, kBlock, lds, st>>>(ds, kp, acc, rng, fb, queues);
 }
 
-void launch_render(uint32_t features, uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st,
+void launch_render(uint32_t features, uint32_t width, uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st,
                    const DevScene &ds, const KernelParams &kp, float4 *acc, uint32_t *rng, float4 *fb,
                    uint32_t *queues)
 {
-#define ORN_CASE(F) \
-    case F: launch_render_f<F>(per_cu, cus, lds, st, ds, kp, acc, rng, fb, queues); break;
+#define ORN_CASE(F)                                                                         \
+    case F:                                                                                 \
+        if (width == 4u)                                                                    \
+            launch_render_f<F, 4>(per_cu, cus, lds, st, ds, kp, acc, rng, fb, queues);      \
+        else                                                                                \
+            launch_render_f<F, 2>(per_cu, cus, lds, st, ds, kp, acc, rng, fb, queues);      \
+        break;
     switch (features & 7u) {
         ORN_CASE(0u)
         ORN_CASE(1u)
@@ -613,6 +723,7 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     ctx->refs++;
     s->refs = 1;
     s->stack_slots = slots;
+    s->trace_width = 2u;
     s->features = desc->textures_len ? kHasTextures : 0u;
     for (size_t i = 0; i < desc->tlas_nodes_len; i++) {
         if (desc->tlas_nodes[i].node_type == 1u) s->features |= kHasSpheres;
@@ -643,6 +754,32 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     const uint4 *u4 = nullptr;
     if ((st = upload(s, reinterpret_cast<const uint4 *>(desc->materials), 2 * desc->materials_len, &u4))) return fail(st);
     ds.materials = u4;
+    // derived BVH4 arrays and the TLAS copy whose mesh leaves point at
+    // their BLAS's BVH4 root
+    {
+        Bvh4Builder t4(desc->tlas_nodes, desc->tlas_nodes_len);
+        const uint32_t root4 = t4.build(static_cast<uint32_t>(desc->tlas_nodes_len - 1));
+        Bvh4Builder b4(desc->blas_nodes, desc->blas_nodes_len);
+        std::vector<uint32_t> blas_ref(desc->blas_nodes_len, 0u);
+        std::vector<orn_bvh_node> leaves(desc->tlas_nodes, desc->tlas_nodes + desc->tlas_nodes_len);
+        for (orn_bvh_node &nd : leaves)
+            if (nd.node_type == 2u) {
+                const uint32_t r = nd.left_or_custom_id;
+                blas_ref[r] = b4.build(r);
+                nd.left_or_custom_id = blas_ref[r];
+            }
+        std::vector<uint32_t> memo(desc->blas_nodes_len, 0xffffffffu);
+        const uint32_t slots4 = std::max(1u, pending4(t4.out, root4, desc->tlas_nodes, &b4.out, &memo, blas_ref.data()));
+        if (slots4 > kMaxStack) return fail(ORN_ERROR_BVH_TOO_DEEP);
+        s->stack_slots4 = slots4;
+        ds.tlas4_root = root4;
+        if ((st = upload(s, reinterpret_cast<const float4 *>(t4.out.data()), 8 * t4.out.size(), &f4))) return fail(st);
+        ds.tlas4 = f4;
+        if ((st = upload(s, reinterpret_cast<const float4 *>(b4.out.data()), 8 * b4.out.size(), &f4))) return fail(st);
+        ds.blas4 = f4;
+        if ((st = upload(s, reinterpret_cast<const float4 *>(leaves.data()), 4 * leaves.size(), &f4))) return fail(st);
+        ds.tlas_leaves = f4;
+    }
     // textures: one texel pool with 16-byte aligned offsets
     std::vector<DevTexture> tex(desc->textures_len);
     uint64_t pool = 0;
@@ -695,7 +832,15 @@ orn_status orn_scene_stack_depth(orn_device_scene *s, uint32_t *depth)
 {
     if (!s) return ORN_ERROR_INVALID_HANDLE;
     if (!depth) return ORN_ERROR_INVALID_VALUE;
-    *depth = s->stack_slots;
+    *depth = s->trace_width == 4u ? s->stack_slots4 : s->stack_slots;
+    return ORN_OK;
+}
+
+orn_status orn_scene_set_trace_width(orn_device_scene *s, uint32_t width)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    if (width != 2u && width != 4u) return ORN_ERROR_INVALID_VALUE;
+    s->trace_width = width;
     return ORN_OK;
 }
 
@@ -767,12 +912,15 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     kp.tiles = kp.tiles_x * blocks_for(t->height, kTile);
     kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
     kp.regroup = t->tuning.regroup_lanes;
-    const size_t lds = size_t(s->stack_slots) * kBlock * sizeof(uint32_t);
+    const uint32_t width = t->tuning.bvh_width == 2u ? 2u : 4u;
+    DevScene ds = s->ds;
+    ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
+    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
     hipStream_t st = s->ctx->stream;
     ORN_TRY(hipEventRecord(t->ev_start, st));
     reset_queues_kernel<<<1, 64, 0, st>>>(t->queues);
-    launch_render(s->features, t->tuning.workgroups_per_cu, static_cast<uint32_t>(s->ctx->compute_units), lds,
-                  st, s->ds, kp, t->acc, t->rng, t->fb, t->queues);
+    launch_render(s->features, width, t->tuning.workgroups_per_cu, static_cast<uint32_t>(s->ctx->compute_units), lds,
+                  st, ds, kp, t->acc, t->rng, t->fb, t->queues);
     ORN_TRY(hipGetLastError());
     ORN_TRY(hipEventRecord(t->ev_stop, st));
     t->timed = true;
@@ -797,7 +945,8 @@ orn_status orn_render(orn_target *t, const orn_constant_params *params, uint32_t
 orn_status orn_target_set_tuning(orn_target *t, const orn_tuning *tuning)
 {
     if (!t) return ORN_ERROR_INVALID_HANDLE;
-    if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64)
+    if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64 ||
+        (tuning->bvh_width != 2u && tuning->bvh_width != 4u))
         return ORN_ERROR_INVALID_VALUE;
     t->tuning = *tuning;
     return ORN_OK;
@@ -877,9 +1026,14 @@ orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, s
     if (e == hipSuccess) e = hipMalloc(&d_out, n * sizeof(orn_hit));
     if (e == hipSuccess) e = hipMemcpyAsync(d_rays, rays, n * 6 * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        const size_t lds = size_t(s->stack_slots) * kBlock * sizeof(uint32_t);
-        trace_kernel<<<blocks_for(static_cast<uint32_t>(n), kBlock), kBlock, lds, st>>>(
-            s->ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
+        DevScene ds = s->ds;
+        ds.stack_slots = s->trace_width == 4u ? s->stack_slots4 : s->stack_slots;
+        const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+        const uint32_t blocks = blocks_for(static_cast<uint32_t>(n), kBlock);
+        if (s->trace_width == 4u)
+            trace_kernel<4><<<blocks, kBlock, lds, st>>>(ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
+        else
+            trace_kernel<2><<<blocks, kBlock, lds, st>>>(ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(orn_hit), hipMemcpyDeviceToHost, st);

@@ -26,10 +26,20 @@ struct DevTexture {
     uint32_t hdr;
 };
 
-// The bound buffers of pathtracer.wgsl:7-17, resident in HBM.
+// BVH4 child references: a BVH4 node index, a BVH2 leaf (kLeafRef | index
+// into the BVH2 array of the same level) or an empty slot.
+constexpr uint32_t kLeafRef = 0x80000000u;
+constexpr uint32_t kEmptyRef = 0xfffffffeu;
+
+// The bound buffers of pathtracer.wgsl:7-17, resident in HBM, plus the
+// derived BVH4 arrays (8 float4 = 128 B per node: children's min x, y, z,
+// max x, y, z as float4 lanes, then the 4 child references).
 struct DevScene {
     const float4 *__restrict__ tlas;        // 4 float4 per node
     const float4 *__restrict__ blas;        // 4 float4 per node
+    const float4 *__restrict__ tlas4;       // BVH4 of the TLAS
+    const float4 *__restrict__ blas4;       // BVH4 of every BLAS
+    const float4 *__restrict__ tlas_leaves; // TLAS copy whose mesh leaves point at BLAS BVH4 roots
     const float4 *__restrict__ normals;     // 1 float4 per normal
     const uint32_t *__restrict__ normal_indices;
     const float2 *__restrict__ uvs;
@@ -41,7 +51,7 @@ struct DevScene {
     uint32_t tlas_len;
     uint32_t textures_len;
     uint32_t stack_slots;  // LDS stack entries per lane
-    uint32_t pad;
+    uint32_t tlas4_root;   // reference of the TLAS root in the BVH4
 };
 
 struct Hit {
@@ -179,13 +189,14 @@ struct Traversal {
     Hit h;
 };
 
+template <uint32_t W>
 __device__ __forceinline__ void trav_begin(const DevScene &sc, const Ray &world, Traversal &tr)
 {
     tr.ray = world;
     tr.inv = safe_invdir(world.d);
     tr.ox = (-world.o) * tr.inv;
     tr.t_max = kTMaxInit;
-    tr.addr = sc.tlas_len - 1u;
+    tr.addr = W == 4u ? sc.tlas4_root : sc.tlas_len - 1u;
     tr.top = 0u;
     tr.material_index = 0u;
     tr.inv_tid = 0u;
@@ -193,16 +204,89 @@ __device__ __forceinline__ void trav_begin(const DevScene &sc, const Ray &world,
     tr.hit = false;
 }
 
+// An empty asm statement that reads and "writes" one register of each of a
+// node's four 16-byte loads: the compiler must have all four loaded at this
+// point and cannot sink any of them into a branch. No instruction is emitted.
+__device__ __forceinline__ void pin_node(float4 &a, float4 &b, float4 &c, float4 &d)
+{
+    asm volatile("" : "+v"(a.x), "+v"(b.x), "+v"(c.x), "+v"(d.x));
+}
+
+// Takes the next pending node off the lane's stack; false when none is left.
 template <uint32_t F>
+__device__ __forceinline__ bool trav_pop(const DevScene &sc, const Ray &world, uint32_t *stack,
+                                         uint32_t stride, Traversal &tr)
+{
+    if (tr.top == 0u) return false;
+    tr.addr = stack[(--tr.top) * stride];
+    if ((F & kHasMeshes) && tr.addr == kFinishedTraverseBlas) {
+        // back in the TLAS: the world-space ray and its reciprocal are
+        // recomputed (the same operations give the same values)
+        tr.in_tlas = true;
+        tr.ray = world;
+        tr.inv = safe_invdir(world.d);
+        tr.ox = (-world.o) * tr.inv;
+        if (tr.top == 0u) return false;
+        tr.addr = stack[(--tr.top) * stride];
+    }
+    return true;
+}
+
+// One BVH4 node: the hit children are pushed in slot order and the last hit
+// one is taken next, so they come off the stack last slot first -- the
+// reference's right-before-left order, two levels at a time.
+__device__ __forceinline__ bool bvh4_node(const float4 *nd, uint32_t *stack, uint32_t stride,
+                                          uint32_t cap, float t_min, Traversal &tr)
+{
+    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+    const uint4 c = *reinterpret_cast<const uint4 *>(nd + 6);
+    const bool h0 = c.x != kEmptyRef &&
+                    aabb_hit(make_float3(lx.x, ly.x, lz.x), make_float3(hx.x, hy.x, hz.x), tr.inv, tr.ox, t_min, tr.t_max);
+    const bool h1 = c.y != kEmptyRef &&
+                    aabb_hit(make_float3(lx.y, ly.y, lz.y), make_float3(hx.y, hy.y, hz.y), tr.inv, tr.ox, t_min, tr.t_max);
+    const bool h2 = c.z != kEmptyRef &&
+                    aabb_hit(make_float3(lx.z, ly.z, lz.z), make_float3(hx.z, hy.z, hz.z), tr.inv, tr.ox, t_min, tr.t_max);
+    const bool h3 = c.w != kEmptyRef &&
+                    aabb_hit(make_float3(lx.w, ly.w, lz.w), make_float3(hx.w, hy.w, hz.w), tr.inv, tr.ox, t_min, tr.t_max);
+    uint32_t next = kEmptyRef;
+    if (h0) next = c.x;
+    if (h1) {
+        if (next != kEmptyRef && tr.top < cap) stack[(tr.top++) * stride] = next;
+        next = c.y;
+    }
+    if (h2) {
+        if (next != kEmptyRef && tr.top < cap) stack[(tr.top++) * stride] = next;
+        next = c.z;
+    }
+    if (h3) {
+        if (next != kEmptyRef && tr.top < cap) stack[(tr.top++) * stride] = next;
+        next = c.w;
+    }
+    if (next == kEmptyRef) return false;
+    tr.addr = next;
+    return true;
+}
+
+template <uint32_t F, uint32_t W>
 __device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const Ray &world,
                                           uint32_t *stack, uint32_t stride, Traversal &tr)
 {
-    const float4 *node =
-        ((F & kHasMeshes) ? (tr.in_tlas ? sc.tlas : sc.blas) : sc.tlas) + 4u * tr.addr;
-    const float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
-    const uint32_t type = as_u32(n2.w);
     const uint32_t cap = sc.stack_slots;
-    if (type == 0u) {
+    if (W == 4u && !(tr.addr & kLeafRef)) {
+        const float4 *nd = ((F & kHasMeshes) && !tr.in_tlas ? sc.blas4 : sc.tlas4) + 8u * tr.addr;
+        if (bvh4_node(nd, stack, stride, cap, t_min, tr)) return true;
+        return trav_pop<F>(sc, world, stack, stride, tr);
+    }
+    const uint32_t idx = W == 4u ? (tr.addr & ~kLeafRef) : tr.addr;
+    const float4 *base = (F & kHasMeshes) && !tr.in_tlas ? sc.blas : (W == 4u ? sc.tlas_leaves : sc.tlas);
+    const float4 *node = base + 4u * idx;
+    float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
+    // Keep the four 16-byte loads of the node together ahead of the type
+    // branch: left alone, the compiler loads the type word first and the
+    // rest inside each branch, two dependent trips to memory per node.
+    pin_node(n0, n1, n2, n3);
+    const uint32_t type = as_u32(n2.w);
+    if (W == 2u && type == 0u) {
         const bool l = aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z),
                                 tr.inv, tr.ox, t_min, tr.t_max);
         const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z),
@@ -252,29 +336,17 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const
             tr.h.v = v;
         }
     }
-    if (tr.top == 0u) return false;
-    tr.addr = stack[(--tr.top) * stride];
-    if ((F & kHasMeshes) && tr.addr == kFinishedTraverseBlas) {
-        // back in the TLAS: the world-space ray and its reciprocal are
-        // recomputed (the same operations give the same values)
-        tr.in_tlas = true;
-        tr.ray = world;
-        tr.inv = safe_invdir(world.d);
-        tr.ox = (-world.o) * tr.inv;
-        if (tr.top == 0u) return false;
-        tr.addr = stack[(--tr.top) * stride];
-    }
-    return true;
+    return trav_pop<F>(sc, world, stack, stride, tr);
 }
 
 // The whole of bvh_hit for one ray.
-template <uint32_t F>
+template <uint32_t F, uint32_t W>
 __device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const Ray &world,
                                         uint32_t *stack, uint32_t stride, Hit &h)
 {
     Traversal tr;
-    trav_begin(sc, world, tr);
-    while (trav_step<F>(sc, t_min, world, stack, stride, tr)) {
+    trav_begin<W>(sc, world, tr);
+    while (trav_step<F, W>(sc, t_min, world, stack, stride, tr)) {
     }
     h = tr.h;
     return tr.hit;
