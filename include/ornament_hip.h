@@ -218,19 +218,23 @@ orn_status orn_render_async(orn_target *target, const orn_constant_params *param
 orn_status orn_target_wait(orn_target *target);
 /* Launch tuning of a target's render kernel; any value gives the same image.
  * regroup_lanes (1..64): a wave interrupts BVH traversal for a shading round
- *   once this many of its 64 lanes have finished their ray (default 32).
+ *   once this many of its 64 lanes have finished their ray (default 48).
  * workgroups_per_cu (0..64): persistent 256-lane workgroups launched per
  *   compute unit; 0 launches as many as can be resident at once.
  * bvh_width (2 or 4): traverse the reference's BVH2 (default) or the derived
  *   BVH4. The BVH4 visits a superset of the reference's nodes, so it returns
  *   the reference's closest hit except where two primitives tie within
  *   rounding at a box boundary (2 of 2,073,600 pixels of a 1080p 32-spp render
- *   of 1,024 instanced icospheres differ, by < 2e-7 relative). */
+ *   of 1,024 instanced icospheres differ, by < 2e-7 relative).
+ * leaf_regroup_lanes (1..64): lanes standing at a BVH leaf (sphere, triangle,
+ *   mesh entry) wait until this many of the wave's lanes do, then test their
+ *   leaves together (default 16). */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
     uint32_t bvh_width;
-    uint32_t reserved[5];
+    uint32_t leaf_regroup_lanes;
+    uint32_t reserved[4];
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);

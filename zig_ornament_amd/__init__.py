@@ -386,18 +386,19 @@ class Target:
     def wait(self):
         check(lib().orn_target_wait(self._h), "orn_target_wait")
 
-    def set_tuning(self, regroup_lanes=32, workgroups_per_cu=0, bvh_width=4):
+    def set_tuning(self, regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16):
         t = N.Tuning()
         t.regroup_lanes = int(regroup_lanes)
         t.workgroups_per_cu = int(workgroups_per_cu)
         t.bvh_width = int(bvh_width)
+        t.leaf_regroup_lanes = int(leaf_regroup_lanes)
         check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
 
     def tuning(self):
         t = N.Tuning()
         check(lib().orn_target_get_tuning(self._h, C.byref(t)), "orn_target_get_tuning")
         return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
-                "bvh_width": t.bvh_width}
+                "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes}
 
     def last_render_ms(self):
         v = C.c_float()

@@ -62,7 +62,7 @@ class SceneDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("regroup_lanes", u32), ("workgroups_per_cu", u32), ("bvh_width", u32),
-                ("reserved", u32 * 5)]
+                ("leaf_regroup_lanes", u32), ("reserved", u32 * 4)]
 
 
 class Hit(C.Structure):

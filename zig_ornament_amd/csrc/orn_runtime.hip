@@ -41,7 +41,8 @@ struct KernelParams {
     uint32_t tiles_x;
     uint32_t tiles;
     uint32_t tiles_per_xcd;
-    uint32_t regroup;  // waiting lanes that trigger a shading round
+    uint32_t regroup;       // waiting lanes that trigger a shading round
+    uint32_t leaf_regroup;  // lanes at a leaf that trigger a leaf round
 };
 
 __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t per_xcd)
@@ -169,9 +170,20 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     };
 
     const uint32_t regroup = kp.regroup;
+    const uint32_t leaf_regroup = kp.leaf_regroup;
     for (;;) {
-        if (mode == kTraverse && !trav_step<F, W>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
+        // internal nodes every iteration; leaves (sphere, triangle, mesh
+        // entry: the long code) in rounds of at least `leaf_regroup` lanes
+        if (mode == kTraverse && !(tr.addr & kLeafRef) &&
+            !trav_internal<F, W>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
             mode = kWaiting;
+        const uint64_t at_leaf = __ballot(mode == kTraverse && (tr.addr & kLeafRef) != 0u);
+        const uint64_t at_inner = __ballot(mode == kTraverse && (tr.addr & kLeafRef) == 0u);
+        if (at_leaf != 0u && (static_cast<uint32_t>(__popcll(at_leaf)) >= leaf_regroup || at_inner == 0u)) {
+            if (mode == kTraverse && (tr.addr & kLeafRef) &&
+                !trav_leaf<F, W>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
+                mode = kWaiting;
+        }
         const uint64_t waiting = __ballot(mode == kWaiting);
         const uint64_t traversing = __ballot(mode == kTraverse);
         if (waiting == 0u && traversing == 0u) break;
@@ -578,9 +590,10 @@ uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 orn_tuning default_tuning()
 {
     orn_tuning t{};
-    t.regroup_lanes = 32;
+    t.regroup_lanes = 48;
     t.workgroups_per_cu = 0;  // as many as can be resident at once
     t.bvh_width = 2;
+    t.leaf_regroup_lanes = 16;
     return t;
 }
 
@@ -754,6 +767,30 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     const uint4 *u4 = nullptr;
     if ((st = upload(s, reinterpret_cast<const uint4 *>(desc->materials), 2 * desc->materials_len, &u4))) return fail(st);
     ds.materials = u4;
+    // BVH2 copies whose child references flag leaves (kLeafRef)
+    {
+        auto flag = [](const orn_bvh_node *arr, uint32_t i) { return arr[i].node_type != 0u ? (i | kLeafRef) : i; };
+        std::vector<orn_bvh_node> t2(desc->tlas_nodes, desc->tlas_nodes + desc->tlas_nodes_len);
+        std::vector<orn_bvh_node> b2(desc->blas_nodes, desc->blas_nodes + desc->blas_nodes_len);
+        for (orn_bvh_node &nd : t2) {
+            if (nd.node_type == 0u) {
+                nd.left_or_custom_id = flag(desc->tlas_nodes, nd.left_or_custom_id);
+                nd.right_or_material_index = flag(desc->tlas_nodes, nd.right_or_material_index);
+            } else if (nd.node_type == 2u) {
+                nd.left_or_custom_id = flag(desc->blas_nodes, nd.left_or_custom_id);
+            }
+        }
+        for (orn_bvh_node &nd : b2)
+            if (nd.node_type == 0u) {
+                nd.left_or_custom_id = flag(desc->blas_nodes, nd.left_or_custom_id);
+                nd.right_or_material_index = flag(desc->blas_nodes, nd.right_or_material_index);
+            }
+        ds.tlas2_root = flag(desc->tlas_nodes, static_cast<uint32_t>(desc->tlas_nodes_len - 1));
+        if ((st = upload(s, reinterpret_cast<const float4 *>(t2.data()), 4 * t2.size(), &f4))) return fail(st);
+        ds.tlas2 = f4;
+        if ((st = upload(s, reinterpret_cast<const float4 *>(b2.data()), 4 * b2.size(), &f4))) return fail(st);
+        ds.blas2 = f4;
+    }
     // derived BVH4 arrays and the TLAS copy whose mesh leaves point at
     // their BLAS's BVH4 root
     {
@@ -912,6 +949,7 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     kp.tiles = kp.tiles_x * blocks_for(t->height, kTile);
     kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
     kp.regroup = t->tuning.regroup_lanes;
+    kp.leaf_regroup = t->tuning.leaf_regroup_lanes;
     const uint32_t width = t->tuning.bvh_width == 2u ? 2u : 4u;
     DevScene ds = s->ds;
     ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
@@ -946,7 +984,8 @@ orn_status orn_target_set_tuning(orn_target *t, const orn_tuning *tuning)
 {
     if (!t) return ORN_ERROR_INVALID_HANDLE;
     if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64 ||
-        (tuning->bvh_width != 2u && tuning->bvh_width != 4u))
+        (tuning->bvh_width != 2u && tuning->bvh_width != 4u) || tuning->leaf_regroup_lanes < 1 ||
+        tuning->leaf_regroup_lanes > 64)
         return ORN_ERROR_INVALID_VALUE;
     t->tuning = *tuning;
     return ORN_OK;

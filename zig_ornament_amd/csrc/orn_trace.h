@@ -40,6 +40,8 @@ struct DevScene {
     const float4 *__restrict__ tlas4;       // BVH4 of the TLAS
     const float4 *__restrict__ blas4;       // BVH4 of every BLAS
     const float4 *__restrict__ tlas_leaves; // TLAS copy whose mesh leaves point at BLAS BVH4 roots
+    const float4 *__restrict__ tlas2;       // TLAS/BLAS copies whose child indices (and mesh
+    const float4 *__restrict__ blas2;       //   leaves' BLAS roots) carry kLeafRef for leaves
     const float4 *__restrict__ normals;     // 1 float4 per normal
     const uint32_t *__restrict__ normal_indices;
     const float2 *__restrict__ uvs;
@@ -52,6 +54,8 @@ struct DevScene {
     uint32_t textures_len;
     uint32_t stack_slots;  // LDS stack entries per lane
     uint32_t tlas4_root;   // reference of the TLAS root in the BVH4
+    uint32_t tlas2_root;   // index of the TLAS root, kLeafRef set when it is a leaf
+    uint32_t pad[3];
 };
 
 struct Hit {
@@ -196,7 +200,7 @@ __device__ __forceinline__ void trav_begin(const DevScene &sc, const Ray &world,
     tr.inv = safe_invdir(world.d);
     tr.ox = (-world.o) * tr.inv;
     tr.t_max = kTMaxInit;
-    tr.addr = W == 4u ? sc.tlas4_root : sc.tlas_len - 1u;
+    tr.addr = W == 4u ? sc.tlas4_root : sc.tlas2_root;
     tr.top = 0u;
     tr.material_index = 0u;
     tr.inv_tid = 0u;
@@ -267,18 +271,47 @@ __device__ __forceinline__ bool bvh4_node(const float4 *nd, uint32_t *stack, uin
     return true;
 }
 
+// Node references carry kLeafRef when they name a leaf (in the BVH4 by
+// construction, in the BVH2 through the flagged copies), so a lane knows
+// whether its next step is an internal node or a leaf before loading it.
+// trav_internal() visits an internal node, trav_leaf() a leaf; both return
+// false once the ray is finished.
 template <uint32_t F, uint32_t W>
-__device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const Ray &world,
-                                          uint32_t *stack, uint32_t stride, Traversal &tr)
+__device__ __forceinline__ bool trav_internal(const DevScene &sc, float t_min, const Ray &world,
+                                              uint32_t *stack, uint32_t stride, Traversal &tr)
 {
     const uint32_t cap = sc.stack_slots;
-    if (W == 4u && !(tr.addr & kLeafRef)) {
-        const float4 *nd = ((F & kHasMeshes) && !tr.in_tlas ? sc.blas4 : sc.tlas4) + 8u * tr.addr;
+    const bool blas = (F & kHasMeshes) && !tr.in_tlas;
+    if (W == 4u) {
+        const float4 *nd = (blas ? sc.blas4 : sc.tlas4) + 8u * tr.addr;
         if (bvh4_node(nd, stack, stride, cap, t_min, tr)) return true;
         return trav_pop<F>(sc, world, stack, stride, tr);
     }
-    const uint32_t idx = W == 4u ? (tr.addr & ~kLeafRef) : tr.addr;
-    const float4 *base = (F & kHasMeshes) && !tr.in_tlas ? sc.blas : (W == 4u ? sc.tlas_leaves : sc.tlas);
+    const float4 *node = (blas ? sc.blas2 : sc.tlas2) + 4u * tr.addr;
+    float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
+    const bool l = aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z), tr.inv,
+                            tr.ox, t_min, tr.t_max);
+    const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z), tr.inv,
+                            tr.ox, t_min, tr.t_max);
+    if (r) {
+        if (l && tr.top < cap) stack[(tr.top++) * stride] = as_u32(n0.w);
+        tr.addr = as_u32(n1.w);
+        return true;
+    }
+    if (l) {
+        tr.addr = as_u32(n0.w);
+        return true;
+    }
+    return trav_pop<F>(sc, world, stack, stride, tr);
+}
+
+template <uint32_t F, uint32_t W>
+__device__ __forceinline__ bool trav_leaf(const DevScene &sc, float t_min, const Ray &world,
+                                          uint32_t *stack, uint32_t stride, Traversal &tr)
+{
+    const uint32_t cap = sc.stack_slots;
+    const uint32_t idx = tr.addr & ~kLeafRef;
+    const float4 *base = (F & kHasMeshes) && !tr.in_tlas ? sc.blas2 : (W == 4u ? sc.tlas_leaves : sc.tlas2);
     const float4 *node = base + 4u * idx;
     float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
     // Keep the four 16-byte loads of the node together ahead of the type
@@ -286,21 +319,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const
     // rest inside each branch, two dependent trips to memory per node.
     pin_node(n0, n1, n2, n3);
     const uint32_t type = as_u32(n2.w);
-    if (W == 2u && type == 0u) {
-        const bool l = aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z),
-                                tr.inv, tr.ox, t_min, tr.t_max);
-        const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z),
-                                tr.inv, tr.ox, t_min, tr.t_max);
-        if (r) {
-            if (l && tr.top < cap) stack[(tr.top++) * stride] = as_u32(n0.w);
-            tr.addr = as_u32(n1.w);
-            return true;
-        }
-        if (l) {
-            tr.addr = as_u32(n0.w);
-            return true;
-        }
-    } else if ((F & kHasSpheres) && type == 1u) {
+    if ((F & kHasSpheres) && type == 1u) {
         const uint32_t tid = as_u32(n3.w) * 2u;
         const float t = sphere_hit(xform_ray(sc.transforms + 4u * tid, tr.ray), t_min, tr.t_max);
         if (t < tr.t_max) {
@@ -337,6 +356,14 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const
         }
     }
     return trav_pop<F>(sc, world, stack, stride, tr);
+}
+
+template <uint32_t F, uint32_t W>
+__device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const Ray &world,
+                                          uint32_t *stack, uint32_t stride, Traversal &tr)
+{
+    if (tr.addr & kLeafRef) return trav_leaf<F, W>(sc, t_min, world, stack, stride, tr);
+    return trav_internal<F, W>(sc, t_min, world, stack, stride, tr);
 }
 
 // The whole of bvh_hit for one ray.
