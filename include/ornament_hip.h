@@ -189,7 +189,8 @@ orn_status orn_scene_validate(const orn_scene_desc *desc, uint32_t *stack_slots)
 orn_status orn_scene_destroy(orn_device_scene *scene);
 /* Besides the reference's BVH2 (bvh.zig), the upload derives a BVH4 from it
  * (each node holds a BVH2 node's grandchildren, in the same order), which the
- * kernels traverse by default; both give the reference's closest hits.
+ * kernels traverse when asked to (orn_tuning.bvh_width, and
+ * orn_scene_set_trace_width for ray queries).
  * Traversal stack slots the BVH of the current trace width needs. */
 orn_status orn_scene_stack_depth(orn_device_scene *scene, uint32_t *depth);
 /* Node width (2 or 4) orn_trace_rays traverses (default 2). */
@@ -252,6 +253,73 @@ orn_status orn_get_rng_state(orn_target *target, uint32_t *dst, size_t pixels);
  * progressive render; either pointer may be NULL to keep the current one). */
 orn_status orn_set_state(orn_target *target, const float *accumulation, const uint32_t *rng,
                          size_t pixels);
+
+/* ------------------------------------------------------------------ */
+/* Batches: many images of many scenes in one persistent launch         */
+/* ------------------------------------------------------------------ */
+
+/* A batch renders n images, each of its own scene, resolution and RNG seed,
+ * in one persistent kernel launch replayed from a captured HIP graph. Each
+ * image's result is the one a separate target would give for the same
+ * scene, seed, params and iterations (bit for bit): a batch is the
+ * reference's render() (wgpu_backend/path_tracer.zig:252-265) run for n
+ * PathTracers at once, and fills the GPU where one small image cannot. */
+typedef struct orn_batch orn_batch;
+typedef struct orn_batch_item {
+    const orn_scene_desc *scene; /* copied at create time */
+    uint32_t width;
+    uint32_t height;
+    uint32_t seed; /* as in orn_target_create */
+    uint32_t reserved;
+} orn_batch_item;
+/* Validates every scene as orn_scene_upload does (the first failing status
+ * is returned) and uploads them as one device scene. */
+orn_status orn_batch_create(orn_ctx *ctx, const orn_batch_item *items, size_t n, orn_batch **out);
+orn_status orn_batch_destroy(orn_batch *batch);
+/* One render() of every image: params[i] and iterations[i] mean what they
+ * mean for orn_render of image i (params[i].width/height must match it). */
+orn_status orn_batch_render(orn_batch *batch, const orn_constant_params *params,
+                            const uint32_t *iterations);
+orn_status orn_batch_render_async(orn_batch *batch, const orn_constant_params *params,
+                                  const uint32_t *iterations);
+orn_status orn_batch_wait(orn_batch *batch);
+orn_status orn_batch_set_tuning(orn_batch *batch, const orn_tuning *tuning);
+orn_status orn_batch_get_tuning(orn_batch *batch, orn_tuning *tuning);
+orn_status orn_batch_last_render_ms(orn_batch *batch, float *ms);
+orn_status orn_batch_get_framebuffer(orn_batch *batch, size_t item, float *dst, size_t pixels);
+orn_status orn_batch_get_accumulation(orn_batch *batch, size_t item, float *dst, size_t pixels);
+orn_status orn_batch_get_rng_state(orn_batch *batch, size_t item, uint32_t *dst, size_t pixels);
+orn_status orn_batch_set_state(orn_batch *batch, size_t item, const float *accumulation,
+                               const uint32_t *rng, size_t pixels);
+
+/* ------------------------------------------------------------------ */
+/* Autotuning (no counterpart in the reference, which has no launch     */
+/* parameters to tune)                                                   */
+/* ------------------------------------------------------------------ */
+
+/* Let the autotuner consider the BVH4 (bvh_width 4), whose closest hits may
+ * differ from the reference's in rare rounding ties (see orn_tuning). */
+#define ORN_TUNE_ALLOW_BVH4 1u
+/* Re-measure even when the cache holds an entry for this scene class. */
+#define ORN_TUNE_IGNORE_CACHE 2u
+
+/* Times renders of `iterations` samples per pixel (a few suffice) under
+ * candidate tunings (coordinate descent over regroup_lanes,
+ * leaf_regroup_lanes, workgroups_per_cu and, with ORN_TUNE_ALLOW_BVH4,
+ * bvh_width), sets the fastest on the target and returns it with its time.
+ * The target's accumulation, RNG state and framebuffer are left as they
+ * were. With a cache_path (a JSON file, created if absent), a tuning stored
+ * for the same device architecture and scene class (scene features, BVH
+ * sizes, pixel count, samples and depth, each bucketed) is applied without
+ * measuring, and a new measurement is stored. Not thread-safe across
+ * processes sharing one cache file. */
+orn_status orn_target_autotune(orn_target *target, const orn_constant_params *params,
+                               uint32_t iterations, uint32_t flags, const char *cache_path,
+                               orn_tuning *chosen, float *ms);
+/* The same for a batch; params and iterations as in orn_batch_render. */
+orn_status orn_batch_autotune(orn_batch *batch, const orn_constant_params *params,
+                              const uint32_t *iterations, uint32_t flags, const char *cache_path,
+                              orn_tuning *chosen, float *ms);
 
 /* ------------------------------------------------------------------ */
 /* Ray queries                                                         */

@@ -364,6 +364,20 @@ class DeviceScene:
             pass
 
 
+def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16):
+    t = N.Tuning()
+    t.regroup_lanes = int(regroup_lanes)
+    t.workgroups_per_cu = int(workgroups_per_cu)
+    t.bvh_width = int(bvh_width)
+    t.leaf_regroup_lanes = int(leaf_regroup_lanes)
+    return t
+
+
+def _tuning_dict(t):
+    return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
+            "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes}
+
+
 class Target:
     """orn_target_create: framebuffer, accumulation and RNG state."""
 
@@ -387,18 +401,23 @@ class Target:
         check(lib().orn_target_wait(self._h), "orn_target_wait")
 
     def set_tuning(self, regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16):
-        t = N.Tuning()
-        t.regroup_lanes = int(regroup_lanes)
-        t.workgroups_per_cu = int(workgroups_per_cu)
-        t.bvh_width = int(bvh_width)
-        t.leaf_regroup_lanes = int(leaf_regroup_lanes)
+        t = _tuning_struct(regroup_lanes, workgroups_per_cu, bvh_width, leaf_regroup_lanes)
         check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
 
     def tuning(self):
         t = N.Tuning()
         check(lib().orn_target_get_tuning(self._h, C.byref(t)), "orn_target_get_tuning")
-        return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
-                "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes}
+        return _tuning_dict(t)
+
+    def autotune(self, params, iterations, allow_bvh4=False, cache_path=None, ignore_cache=False):
+        """orn_target_autotune: times candidate tunings on renders of
+        `iterations` spp, keeps the fastest; returns (tuning dict, ms)."""
+        t, ms = N.Tuning(), C.c_float()
+        flags = (N.TUNE_ALLOW_BVH4 if allow_bvh4 else 0) | (N.TUNE_IGNORE_CACHE if ignore_cache else 0)
+        path = cache_path.encode() if cache_path else None
+        check(lib().orn_target_autotune(self._h, C.byref(params), int(iterations), flags, path,
+                                        C.byref(t), C.byref(ms)), "orn_target_autotune")
+        return _tuning_dict(t), ms.value
 
     def last_render_ms(self):
         v = C.c_float()
@@ -429,6 +448,110 @@ class Target:
     def destroy(self):
         if self._h:
             lib().orn_target_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Batch:
+    """orn_batch_create: many images of many scenes rendered by one launch.
+
+    `items` is a list of (Bvh arrays, width, height, seed)."""
+
+    def __init__(self, device, items):
+        keep = []
+        arr = (N.BatchItem * len(items))()
+        descs = []
+        for i, (arrays, w, h, seed) in enumerate(items):
+            d, k = scene_desc_arrays(arrays)
+            keep.append(k)
+            descs.append(d)
+            arr[i].scene = C.addressof(d)
+            arr[i].width, arr[i].height, arr[i].seed = int(w), int(h), int(seed)
+        hnd = C.c_void_p()
+        check(lib().orn_batch_create(device.handle, arr, len(items), C.byref(hnd)),
+              "orn_batch_create")
+        self._h = hnd
+        self.device = device
+        self.sizes = [(int(w), int(h)) for _, w, h, _ in items]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def _args(self, params, iterations):
+        if len(params) != len(self) or len(iterations) != len(self):
+            raise ValueError("one params and one iteration count per image")
+        p = (N.ConstantParams * len(self))(*params)
+        it = (C.c_uint32 * len(self))(*[int(i) for i in iterations])
+        return p, it
+
+    def render(self, params, iterations, wait=True):
+        p, it = self._args(params, iterations)
+        fn = lib().orn_batch_render if wait else lib().orn_batch_render_async
+        check(fn(self._h, p, it), "orn_batch_render")
+
+    def wait(self):
+        check(lib().orn_batch_wait(self._h), "orn_batch_wait")
+
+    def set_tuning(self, **kw):
+        t = _tuning_struct(**kw)
+        check(lib().orn_batch_set_tuning(self._h, C.byref(t)), "orn_batch_set_tuning")
+
+    def tuning(self):
+        t = N.Tuning()
+        check(lib().orn_batch_get_tuning(self._h, C.byref(t)), "orn_batch_get_tuning")
+        return _tuning_dict(t)
+
+    def autotune(self, params, iterations, allow_bvh4=False, cache_path=None, ignore_cache=False):
+        p, it = self._args(params, iterations)
+        t, ms = N.Tuning(), C.c_float()
+        flags = (N.TUNE_ALLOW_BVH4 if allow_bvh4 else 0) | (N.TUNE_IGNORE_CACHE if ignore_cache else 0)
+        path = cache_path.encode() if cache_path else None
+        check(lib().orn_batch_autotune(self._h, p, it, flags, path, C.byref(t), C.byref(ms)),
+              "orn_batch_autotune")
+        return _tuning_dict(t), ms.value
+
+    def last_render_ms(self):
+        v = C.c_float()
+        check(lib().orn_batch_last_render_ms(self._h, C.byref(v)), "orn_batch_last_render_ms")
+        return v.value
+
+    def _read(self, fn, item, dtype, comps, name):
+        w, h = self.sizes[item]
+        out = np.empty((w * h, comps) if comps > 1 else w * h, dtype=dtype)
+        check(fn(self._h, int(item), _ptr(out), w * h), name)
+        return out
+
+    def framebuffer(self, item):
+        return self._read(lib().orn_batch_get_framebuffer, item, np.float32, 4,
+                          "orn_batch_get_framebuffer")
+
+    def accumulation(self, item):
+        return self._read(lib().orn_batch_get_accumulation, item, np.float32, 4,
+                          "orn_batch_get_accumulation")
+
+    def rng_state(self, item):
+        return self._read(lib().orn_batch_get_rng_state, item, np.uint32, 1,
+                          "orn_batch_get_rng_state")
+
+    def set_state(self, item, accumulation=None, rng=None):
+        w, h = self.sizes[item]
+        a = None if accumulation is None else np.ascontiguousarray(accumulation, np.float32)
+        r = None if rng is None else np.ascontiguousarray(rng, np.uint32)
+        check(lib().orn_batch_set_state(self._h, int(item), _ptr(a), _ptr(r), w * h),
+              "orn_batch_set_state")
+
+    def destroy(self):
+        if self._h:
+            lib().orn_batch_destroy(self._h)
         self._h = None
 
     def __del__(self):

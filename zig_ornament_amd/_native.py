@@ -65,6 +65,15 @@ class Tuning(C.Structure):
                 ("leaf_regroup_lanes", u32), ("reserved", u32 * 4)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", u32), ("height", u32), ("seed", u32),
+                ("reserved", u32)]
+
+
+TUNE_ALLOW_BVH4 = 1
+TUNE_IGNORE_CACHE = 2
+
+
 class Hit(C.Structure):
     _fields_ = [("hit", u32), ("t", f32), ("material_index", u32), ("node_type", u32),
                 ("inverted_transform_id", u32), ("tri_id", u32), ("uv", f32 * 2)]
@@ -107,6 +116,20 @@ DEVICE_API = [
     ("orn_trace_rays", st, [vp, f32, vp, sz, vp]),
     ("orn_eval_math", st, [vp, C.c_int, vp, vp, vp, sz]),
     ("orn_calibration_copy", st, [vp, sz, fp]),
+    ("orn_batch_create", st, [vp, P(BatchItem), sz, P(vp)]),
+    ("orn_batch_destroy", st, [vp]),
+    ("orn_batch_render", st, [vp, P(ConstantParams), up]),
+    ("orn_batch_render_async", st, [vp, P(ConstantParams), up]),
+    ("orn_batch_wait", st, [vp]),
+    ("orn_batch_set_tuning", st, [vp, P(Tuning)]),
+    ("orn_batch_get_tuning", st, [vp, P(Tuning)]),
+    ("orn_batch_last_render_ms", st, [vp, fp]),
+    ("orn_batch_get_framebuffer", st, [vp, sz, vp, sz]),
+    ("orn_batch_get_accumulation", st, [vp, sz, vp, sz]),
+    ("orn_batch_get_rng_state", st, [vp, sz, vp, sz]),
+    ("orn_batch_set_state", st, [vp, sz, vp, vp, sz]),
+    ("orn_target_autotune", st, [vp, P(ConstantParams), u32, u32, C.c_char_p, P(Tuning), fp]),
+    ("orn_batch_autotune", st, [vp, P(ConstantParams), up, u32, C.c_char_p, P(Tuning), fp]),
 ]
 
 HOST_API = [

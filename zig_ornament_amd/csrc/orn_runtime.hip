@@ -20,7 +20,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <new>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ornament_hip.h"
@@ -36,30 +39,49 @@ constexpr uint32_t kXcds = 8;
 constexpr uint32_t kMaxStack = 63;  // the reference's private stack has 64 slots
 
 struct KernelParams {
-    orn_constant_params p;
+    orn_constant_params p;  // single image; unused by batch launches
     uint32_t iterations;
     uint32_t tiles_x;
     uint32_t tiles;
     uint32_t tiles_per_xcd;
     uint32_t regroup;       // waiting lanes that trigger a shading round
     uint32_t leaf_regroup;  // lanes at a leaf that trigger a leaf round
+    uint32_t root;          // TLAS root reference (single image)
+    uint32_t pad;
 };
 
-__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t per_xcd)
-{
-    return (b % kXcds) * per_xcd + b / kXcds;
-}
+// One image of a batch launch. All images of a batch share one device scene
+// (their arrays concatenated) and one pool of pixel buffers.
+struct BatchItem {
+    orn_constant_params p;  // as in a single render() of this image
+    uint32_t iterations;
+    uint32_t tiles_x;
+    uint32_t px_base;  // first pixel of the image in the pooled buffers
+    uint32_t root;     // TLAS root reference of the image's scene
+    uint32_t tex_base; // first texture of the image's scene in the shared table
+    uint32_t pad[3];
+};
+static_assert(sizeof(BatchItem) == 176, "BatchItem layout");
+
+// Tile work list of a batch launch: band b owns entries
+// [band_begin[b], band_begin[b] + band_tiles[b]) of `tiles`, each an
+// (image, tile of the image) pair.
+struct BatchTiles {
+    uint32_t band_begin[kXcds];
+    uint32_t band_tiles[kXcds];
+    uint2 tiles[1];  // band_begin[kXcds-1] + band_tiles[kXcds-1] entries
+};
 
 enum LaneMode : uint32_t { kTraverse = 0u, kWaiting = 1u, kDone = 2u };
 
 // pathtracer.wgsl:64-79 + 95-161 + 81-93, every pixel over all iterations.
 //
-// Persistent lanes: the image is cut into 16x16 tiles, the tiles into one
-// contiguous band per XCD (workgroup b runs on XCD b % 8), and each band is
-// a work queue of pixels. A lane renders one pixel's samples back to back
-// (the per-pixel RNG chain is serial), writes the pixel out and takes the
-// next pixel of its band's queue, so no lane idles while a slow pixel of its
-// wave finishes.
+// Persistent lanes: the work is cut into 16x16 pixel tiles, the tiles into
+// one band per XCD (workgroup b runs on XCD b % 8), and each band is a work
+// queue of pixels. A lane renders one pixel's samples back to back (the
+// per-pixel RNG chain is serial), writes the pixel out and takes the next
+// pixel of its wave's band; a wave whose band is empty moves on to the next
+// band, so uneven bands even out.
 // Within a wave, a lane alternates between traversing its current ray one
 // node per loop iteration and a shading round (hit shading, accumulation,
 // next camera path, next pixel). A lane whose traversal ended waits; the wave
@@ -67,36 +89,26 @@ enum LaneMode : uint32_t { kTraverse = 0u, kWaiting = 1u, kDone = 2u };
 // traversing), so node visits run with most lanes busy and the long shading
 // code runs for a batch of lanes at once. Each pixel's own sequence of
 // operations is the reference's, so results do not depend on the schedule.
-template <uint32_t F, uint32_t W>
+//
+// BATCH: the tiles come from a work list over many images (BatchTiles), and
+// each lane reads its image's parameters from the BatchItem table; the
+// pixel buffers are pools indexed by px_base + pixel.
+template <uint32_t F, uint32_t W, bool BATCH>
 __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParams kp,
+                                                        const BatchItem *__restrict__ items,
+                                                        const BatchTiles *__restrict__ work,
                                                         float4 *__restrict__ acc_buf,
                                                         uint32_t *__restrict__ rng_buf,
                                                         float4 *__restrict__ fb,
                                                         uint32_t *__restrict__ queues)
 {
     extern __shared__ uint32_t lds_stack[];
-    const orn_constant_params &p = kp.p;
     uint32_t *stack = lds_stack + threadIdx.x;
-    const uint32_t band = blockIdx.x % kXcds;
-    const uint32_t band_first_tile = band * kp.tiles_per_xcd;
-    const uint32_t band_tiles =
-        band_first_tile >= kp.tiles ? 0u : min(kp.tiles_per_xcd, kp.tiles - band_first_tile);
-    const uint32_t band_pixels = band_tiles * (kTile * kTile);
-    uint32_t *queue = queues + band;
     const uint32_t lane = threadIdx.x % 64u;
 
-    const orn_camera &cam = p.camera;
-    const f3 cam_origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
-    const f3 cam_llc = mk(cam.lower_left_corner[0], cam.lower_left_corner[1], cam.lower_left_corner[2]);
-    const f3 cam_h = mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2]);
-    const f3 cam_v = mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]);
-    const f3 cam_u = mk(cam.u[0], cam.u[1], cam.u[2]);
-    const f3 cam_vv = mk(cam.v[0], cam.v[1], cam.v[2]);
-    const float w_den = static_cast<float>(p.width - 1u);
-    const float h_den = static_cast<float>(p.height - 1u);
-    const uint32_t spp = kp.iterations > 1u ? kp.iterations : 1u;
-
-    uint32_t x = 0, y = 0, pix = 0;
+    uint32_t x = 0, y = 0;
+    uint32_t pix = 0;   // pixel index in the pixel buffers
+    uint32_t item = 0;  // image of the lane's pixel (BATCH)
     uint32_t rng = 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float cur = 0.0f;
@@ -107,21 +119,40 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     Traversal tr;
     uint32_t mode = kWaiting;
     bool need_pixel = true;
+    // per-image values a lane keeps while it works on a pixel (BATCH); a
+    // single image takes them from the kernel arguments
+    float lane_t_min = 0.0f;
+    uint32_t lane_depth = 0, lane_root = 0;
+    auto t_min = [&]() { return BATCH ? lane_t_min : kp.p.ray_cast_epsilon; };
+    auto depth = [&]() { return BATCH ? lane_depth : kp.p.depth; };
+    auto root = [&]() { return BATCH ? lane_root : kp.root; };
 
-    // camera ray of a new sample: 2 jitter draws + 2 lens draws
+    auto params = [&]() -> const orn_constant_params & {
+        if constexpr (BATCH) return items[item].p;
+        else return kp.p;
+    };
+
+    // camera ray of a new sample (camera.wgsl:12-23): 2 jitter draws + 2 lens draws
     auto start_sample = [&]() {
-        const float u = (static_cast<float>(x) + rng_f32(rng)) / w_den;
-        const float v = (static_cast<float>(y) + rng_f32(rng)) / h_den;
+        const orn_constant_params &p = params();
+        const orn_camera &cam = p.camera;
+        const float u = (static_cast<float>(x) + rng_f32(rng)) / static_cast<float>(p.width - 1u);
+        const float v = (static_cast<float>(y) + rng_f32(rng)) / static_cast<float>(p.height - 1u);
         const f3 disk = random_in_unit_disk(rng);
         const f3 rd = cam.lens_radius * disk;
-        const f3 offset = rd.x * cam_u + rd.y * cam_vv;
-        ray.o = cam_origin + offset;
-        ray.d = ((cam_llc + u * cam_h) + v * cam_v - cam_origin) - offset;
+        const f3 offset = rd.x * mk(cam.u[0], cam.u[1], cam.u[2]) + rd.y * mk(cam.v[0], cam.v[1], cam.v[2]);
+        const f3 origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+        ray.o = origin + offset;
+        ray.d = ((mk(cam.lower_left_corner[0], cam.lower_left_corner[1], cam.lower_left_corner[2]) +
+                  u * mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2])) +
+                 v * mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]) - origin) -
+                offset;
         thr = mk(1.0f, 1.0f, 1.0f);
         bounce = 0;
     };
     // pathtracer.wgsl:81-93 and the state write-back of one finished pixel
     auto finish_pixel = [&]() {
+        const orn_constant_params &p = params();
         acc_buf[pix] = acc;
         rng_buf[pix] = rng;
         float4 o;
@@ -132,8 +163,9 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         o.x = pow_f(o.x, p.inverted_gamma);
         o.y = pow_f(o.y, p.inverted_gamma);
         o.z = pow_f(o.z, p.inverted_gamma);
-        const uint32_t dst = p.flip_y < 1u ? pix : p.width * (p.height - y - 1u) + x;
-        fb[dst] = o;
+        const uint32_t row = p.flip_y < 1u ? y : p.height - y - 1u;
+        const uint32_t base = BATCH ? items[item].px_base : 0u;
+        fb[base + row * p.width + x] = o;
     };
     // Accumulate finished samples, finish samples that need no traversal
     // (depth exhausted), and set up the next traversal; a finished pixel
@@ -158,30 +190,39 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                 }
                 start_sample();
             }
-            if (bounce >= p.depth) {  // depth exhausted: the throughput is returned
+            if (bounce >= depth()) {  // depth exhausted: the throughput is returned
                 color = thr;
                 done = true;
                 continue;
             }
-            trav_begin<W>(sc, ray, tr);
+            trav_begin(root(), ray, tr);
             mode = kTraverse;
             return;
         }
     };
+    // pixels per band
+    auto band_pixels = [&](uint32_t b) -> uint32_t {
+        if constexpr (BATCH) return work->band_tiles[b] * (kTile * kTile);
+        const uint32_t first = b * kp.tiles_per_xcd;
+        return first >= kp.tiles ? 0u : min(kp.tiles_per_xcd, kp.tiles - first) * (kTile * kTile);
+    };
 
+    uint32_t band = blockIdx.x % kXcds;  // wave-uniform
+    uint32_t bands_left = kXcds - 1u;  // bands to visit after this one
+    uint32_t band_px = band_pixels(band);
     const uint32_t regroup = kp.regroup;
     const uint32_t leaf_regroup = kp.leaf_regroup;
     for (;;) {
         // internal nodes every iteration; leaves (sphere, triangle, mesh
         // entry: the long code) in rounds of at least `leaf_regroup` lanes
         if (mode == kTraverse && !(tr.addr & kLeafRef) &&
-            !trav_internal<F, W>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
+            !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr))
             mode = kWaiting;
         const uint64_t at_leaf = __ballot(mode == kTraverse && (tr.addr & kLeafRef) != 0u);
         const uint64_t at_inner = __ballot(mode == kTraverse && (tr.addr & kLeafRef) == 0u);
         if (at_leaf != 0u && (static_cast<uint32_t>(__popcll(at_leaf)) >= leaf_regroup || at_inner == 0u)) {
             if (mode == kTraverse && (tr.addr & kLeafRef) &&
-                !trav_leaf<F, W>(sc, p.ray_cast_epsilon, ray, stack, kBlock, tr))
+                !trav_leaf<F, W>(sc, t_min(), ray, stack, kBlock, tr))
                 mode = kWaiting;
         }
         const uint64_t waiting = __ballot(mode == kWaiting);
@@ -199,7 +240,9 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
             } else {
                 Ray next;
                 f3 w;
-                if (shade<F>(sc, p.textures_count, ray, tr.h, rng, next, w)) {
+                const uint32_t tex_count = params().textures_count;
+                const uint32_t tex_base = BATCH ? items[item].tex_base : 0u;
+                if (shade<F>(sc, tex_count, tex_base, ray, tr.h, rng, next, w)) {
                     ray = next;
                     thr = thr * w;
                     next_traversal(false, thr);
@@ -208,36 +251,64 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                 }
             }
         }
-        // lanes without a pixel take the next ones of the band's queue, one
-        // atomic per wave; pixels of partial edge tiles are skipped
+        // lanes without a pixel take the next ones of the wave's band, one
+        // atomic per wave; pixels of partial edge tiles are skipped; a band
+        // that runs out sends the wave on to the next one
         for (;;) {
             const uint64_t needy = __ballot(mode == kWaiting && need_pixel);
             if (needy == 0u) break;
+            const int leader = __ffsll(static_cast<unsigned long long>(needy)) - 1;
+            const uint32_t count = static_cast<uint32_t>(__popcll(needy));
             uint32_t base = 0;
-            if (lane == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(needy)) - 1))
-                base = atomicAdd(queue, static_cast<uint32_t>(__popcll(needy)));
-            base = __shfl(base, __ffsll(static_cast<unsigned long long>(needy)) - 1);
+            if (lane == static_cast<uint32_t>(leader)) base = atomicAdd(queues + band, count);
+            base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), leader));
             if (mode == kWaiting && need_pixel) {
-                const uint32_t q =
-                    base + static_cast<uint32_t>(__popcll(needy & ((1ull << lane) - 1ull)));
-                if (q >= band_pixels) {
-                    mode = kDone;
+                const uint32_t q = base + static_cast<uint32_t>(__popcll(needy & ((1ull << lane) - 1ull)));
+                if (q >= band_px) {
+                    if (bands_left == 0u) mode = kDone;  // every band has run out
                 } else {
-                    const uint32_t tile = band_first_tile + q / (kTile * kTile);
                     const uint32_t within = q % (kTile * kTile);
-                    x = (tile % kp.tiles_x) * kTile + within % kTile;
-                    y = (tile / kp.tiles_x) * kTile + within / kTile;
-                    if (x < p.width && y < p.height) {
-                        pix = y * p.width + x;
+                    uint32_t tile, tiles_x, w, h;
+                    if constexpr (BATCH) {
+                        const uint2 e = work->tiles[work->band_begin[band] + q / (kTile * kTile)];
+                        item = e.x;
+                        tile = e.y;
+                        tiles_x = items[item].tiles_x;
+                        w = items[item].p.width;
+                        h = items[item].p.height;
+                    } else {
+                        tile = band * kp.tiles_per_xcd + q / (kTile * kTile);
+                        tiles_x = kp.tiles_x;
+                        w = kp.p.width;
+                        h = kp.p.height;
+                    }
+                    x = (tile % tiles_x) * kTile + within % kTile;
+                    y = (tile / tiles_x) * kTile + within / kTile;
+                    if (x < w && y < h) {
+                        const orn_constant_params &p = params();
+                        pix = y * w + x;
+                        uint32_t spp = kp.iterations;
+                        if constexpr (BATCH) {
+                            pix += items[item].px_base;
+                            spp = items[item].iterations;
+                            lane_t_min = p.ray_cast_epsilon;
+                            lane_depth = p.depth;
+                            lane_root = items[item].root;
+                        }
                         rng = rng_buf[pix];
                         acc = acc_buf[pix];
                         cur = p.current_iteration;
-                        left = spp;
+                        left = spp > 1u ? spp : 1u;
                         need_pixel = false;
                         start_sample();
                         next_traversal(false, thr);
                     }
                 }
+            }
+            if (base + count > band_px && bands_left != 0u) {  // the band ran out (wave-uniform)
+                band = band + 1u == kXcds ? 0u : band + 1u;
+                bands_left--;
+                band_px = band_pixels(band);
             }
         }
     }
@@ -324,6 +395,7 @@ struct orn_ctx {
     int device;
     hipStream_t stream;
     char name[256];
+    char arch[64];  // gcnArchName, the autotuner's cache key
     int compute_units;
     int refs;
 };
@@ -336,6 +408,7 @@ struct orn_device_scene {
     uint32_t stack_slots4;  // BVH4 traversal
     uint32_t trace_width;   // node width orn_trace_rays uses
     uint32_t features;      // SceneFeatures present in the arrays
+    uint32_t blas_len;      // BVH2 BLAS nodes
     int refs;
 };
 
@@ -597,37 +670,48 @@ orn_tuning default_tuning()
     return t;
 }
 
+// Buffers and sizes of one render launch.
+struct LaunchArgs {
+    KernelParams kp;
+    const BatchItem *items;
+    const BatchTiles *work;
+    float4 *acc;
+    uint32_t *rng;
+    float4 *fb;
+    uint32_t *queues;
+    uint32_t total_tiles;
+};
+
 // One instantiation per combination of scene features.
 // Persistent grid: workgroups_per_cu per compute unit (0: as many as can be
 // resident at once), a multiple of the XCD count, at most one per tile.
-template <uint32_t F, uint32_t W>
-void launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
-                     const KernelParams &kp, float4 *acc, uint32_t *rng, float4 *fb, uint32_t *queues)
+template <uint32_t F, uint32_t W, bool BATCH>
+hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
+                           const LaunchArgs &a)
 {
     if (per_cu == 0) {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F, W>, kBlock, lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F, W, BATCH>, kBlock, lds) !=
+                hipSuccess ||
             blocks < 1)
             blocks = 1;
         per_cu = static_cast<uint32_t>(blocks);
     }
     uint32_t grid = cus * per_cu;
-    grid = std::min(grid, kp.tiles_per_xcd * kXcds);
+    grid = std::min(grid, blocks_for(a.total_tiles, kXcds) * kXcds);
     grid = std::max(kXcds, grid / kXcds * kXcds);
-    render_kernel<F, W><<<grid, kBlock, lds, st>>>(ds, kp, acc, rng, fb, queues);
+    render_kernel<F, W, BATCH><<<grid, kBlock, lds, st>>>(ds, a.kp, a.items, a.work, a.acc, a.rng, a.fb, a.queues);
+    return hipGetLastError();
 }
 
-void launch_render(uint32_t features, uint32_t width, uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st,
-                   const DevScene &ds, const KernelParams &kp, float4 *acc, uint32_t *rng, float4 *fb,
-                   uint32_t *queues)
+template <bool BATCH>
+hipError_t launch_render(uint32_t features, uint32_t width, uint32_t per_cu, uint32_t cus, size_t lds,
+                         hipStream_t st, const DevScene &ds, const LaunchArgs &a)
 {
-#define ORN_CASE(F)                                                                         \
-    case F:                                                                                 \
-        if (width == 4u)                                                                    \
-            launch_render_f<F, 4>(per_cu, cus, lds, st, ds, kp, acc, rng, fb, queues);      \
-        else                                                                                \
-            launch_render_f<F, 2>(per_cu, cus, lds, st, ds, kp, acc, rng, fb, queues);      \
-        break;
+#define ORN_CASE(F)                                                                      \
+    case F:                                                                              \
+        return width == 4u ? launch_render_f<F, 4, BATCH>(per_cu, cus, lds, st, ds, a)  \
+                           : launch_render_f<F, 2, BATCH>(per_cu, cus, lds, st, ds, a);
     switch (features & 7u) {
         ORN_CASE(0u)
         ORN_CASE(1u)
@@ -639,6 +723,7 @@ void launch_render(uint32_t features, uint32_t width, uint32_t per_cu, uint32_t 
         ORN_CASE(7u)
     }
 #undef ORN_CASE
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -693,6 +778,7 @@ orn_status orn_device_open(int device, orn_ctx **out)
         return from_hip(e);
     }
     std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     c->compute_units = prop.multiProcessorCount;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -721,14 +807,18 @@ orn_status orn_device_info(orn_ctx *ctx, char *name, size_t name_len, int *compu
     return ORN_OK;
 }
 
-orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device_scene **out)
+}  // extern "C"
+
+namespace {
+
+// Uploads validated arrays. `roots` are the TLAS root indices of the scenes
+// the arrays hold (one, or one per image of a batch); their BVH2 and BVH4
+// root references are returned in root2/root4. `slots` is the largest BVH2
+// stack bound of those scenes.
+orn_status upload_scene(orn_ctx *ctx, const orn_scene_desc *desc, uint32_t slots, const uint32_t *roots,
+                        size_t n_roots, uint32_t *root2, uint32_t *root4, orn_device_scene **out)
 {
-    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
-    if (!desc || !out) return ORN_ERROR_INVALID_VALUE;
-    *out = nullptr;
-    uint32_t slots = 0;
-    orn_status st = validate(desc, &slots);
-    if (st != ORN_OK) return st;
+    orn_status st = ORN_OK;
     ORN_TRY(hipSetDevice(ctx->device));
     orn_device_scene *s = new (std::nothrow) orn_device_scene();
     if (!s) return ORN_ERROR_OUT_OF_MEMORY;
@@ -736,6 +826,7 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     ctx->refs++;
     s->refs = 1;
     s->stack_slots = slots;
+    s->blas_len = static_cast<uint32_t>(desc->blas_nodes_len);
     s->trace_width = 2u;
     s->features = desc->textures_len ? kHasTextures : 0u;
     for (size_t i = 0; i < desc->tlas_nodes_len; i++) {
@@ -785,7 +876,8 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
                 nd.left_or_custom_id = flag(desc->blas_nodes, nd.left_or_custom_id);
                 nd.right_or_material_index = flag(desc->blas_nodes, nd.right_or_material_index);
             }
-        ds.tlas2_root = flag(desc->tlas_nodes, static_cast<uint32_t>(desc->tlas_nodes_len - 1));
+        for (size_t i = 0; i < n_roots; i++) root2[i] = flag(desc->tlas_nodes, roots[i]);
+        ds.tlas2_root = root2[0];
         if ((st = upload(s, reinterpret_cast<const float4 *>(t2.data()), 4 * t2.size(), &f4))) return fail(st);
         ds.tlas2 = f4;
         if ((st = upload(s, reinterpret_cast<const float4 *>(b2.data()), 4 * b2.size(), &f4))) return fail(st);
@@ -795,7 +887,7 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     // their BLAS's BVH4 root
     {
         Bvh4Builder t4(desc->tlas_nodes, desc->tlas_nodes_len);
-        const uint32_t root4 = t4.build(static_cast<uint32_t>(desc->tlas_nodes_len - 1));
+        for (size_t i = 0; i < n_roots; i++) root4[i] = t4.build(roots[i]);
         Bvh4Builder b4(desc->blas_nodes, desc->blas_nodes_len);
         std::vector<uint32_t> blas_ref(desc->blas_nodes_len, 0u);
         std::vector<orn_bvh_node> leaves(desc->tlas_nodes, desc->tlas_nodes + desc->tlas_nodes_len);
@@ -806,10 +898,12 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
                 nd.left_or_custom_id = blas_ref[r];
             }
         std::vector<uint32_t> memo(desc->blas_nodes_len, 0xffffffffu);
-        const uint32_t slots4 = std::max(1u, pending4(t4.out, root4, desc->tlas_nodes, &b4.out, &memo, blas_ref.data()));
+        uint32_t slots4 = 1;
+        for (size_t i = 0; i < n_roots; i++)
+            slots4 = std::max(slots4, pending4(t4.out, root4[i], desc->tlas_nodes, &b4.out, &memo, blas_ref.data()));
         if (slots4 > kMaxStack) return fail(ORN_ERROR_BVH_TOO_DEEP);
         s->stack_slots4 = slots4;
-        ds.tlas4_root = root4;
+        ds.tlas4_root = root4[0];
         if ((st = upload(s, reinterpret_cast<const float4 *>(t4.out.data()), 8 * t4.out.size(), &f4))) return fail(st);
         ds.tlas4 = f4;
         if ((st = upload(s, reinterpret_cast<const float4 *>(b4.out.data()), 8 * b4.out.size(), &f4))) return fail(st);
@@ -847,6 +941,23 @@ orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device
     ds.stack_slots = slots;
     *out = s;
     return ORN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+orn_status orn_scene_upload(orn_ctx *ctx, const orn_scene_desc *desc, orn_device_scene **out)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    if (!desc || !out) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    uint32_t slots = 0;
+    orn_status st = validate(desc, &slots);
+    if (st != ORN_OK) return st;
+    const uint32_t root = static_cast<uint32_t>(desc->tlas_nodes_len - 1);
+    uint32_t root2 = 0, root4 = 0;
+    return upload_scene(ctx, desc, slots, &root, 1, &root2, &root4, out);
 }
 
 orn_status orn_scene_validate(const orn_scene_desc *desc, uint32_t *stack_slots)
@@ -951,15 +1062,17 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     kp.regroup = t->tuning.regroup_lanes;
     kp.leaf_regroup = t->tuning.leaf_regroup_lanes;
     const uint32_t width = t->tuning.bvh_width == 2u ? 2u : 4u;
+    kp.root = width == 4u ? s->ds.tlas4_root : s->ds.tlas2_root;
+    kp.pad = 0;
     DevScene ds = s->ds;
     ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
     const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
     hipStream_t st = s->ctx->stream;
+    LaunchArgs a{kp, nullptr, nullptr, t->acc, t->rng, t->fb, t->queues, kp.tiles};
     ORN_TRY(hipEventRecord(t->ev_start, st));
     reset_queues_kernel<<<1, 64, 0, st>>>(t->queues);
-    launch_render(s->features, width, t->tuning.workgroups_per_cu, static_cast<uint32_t>(s->ctx->compute_units), lds,
-                  st, ds, kp, t->acc, t->rng, t->fb, t->queues);
-    ORN_TRY(hipGetLastError());
+    ORN_TRY(launch_render<false>(s->features, width, t->tuning.workgroups_per_cu,
+                                 static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a));
     ORN_TRY(hipEventRecord(t->ev_stop, st));
     t->timed = true;
     return ORN_OK;
@@ -1138,6 +1251,787 @@ orn_status orn_calibration_copy(orn_ctx *ctx, size_t bytes, float *ms)
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return from_hip(e);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches
+// ===========================================================================
+
+namespace {
+
+// The arrays of many scenes concatenated into one, with every index
+// rebased: node links, BLAS roots, leaf transform and material ids,
+// triangle ids, normal and uv indices, texture ids.
+struct MergedScene {
+    std::vector<orn_bvh_node> tlas, blas;
+    std::vector<float> normals, uvs, transforms;
+    std::vector<uint32_t> normal_indices, uv_indices;
+    std::vector<orn_material> materials;
+    std::vector<orn_texture_desc> textures;
+    std::vector<uint32_t> roots;     // TLAS root of each scene
+    std::vector<uint32_t> tex_base;  // first texture of each scene
+    orn_scene_desc desc{};
+};
+
+orn_status merge_scenes(const orn_batch_item *items, size_t n, MergedScene &m, uint32_t *slots)
+{
+    *slots = 1;
+    for (size_t i = 0; i < n; i++) {
+        const orn_scene_desc *d = items[i].scene;
+        if (!d) return ORN_ERROR_INVALID_VALUE;
+        uint32_t sl = 0;
+        const orn_status st = validate(d, &sl);
+        if (st != ORN_OK) return st;
+        *slots = std::max(*slots, sl);
+        const uint32_t tlas_base = static_cast<uint32_t>(m.tlas.size());
+        const uint32_t blas_base = static_cast<uint32_t>(m.blas.size());
+        const uint32_t mat_base = static_cast<uint32_t>(m.materials.size());
+        const uint32_t tid_base = static_cast<uint32_t>(m.transforms.size() / 32);  // [inverse, model] pairs
+        const uint32_t tri_base = static_cast<uint32_t>(m.normal_indices.size() / 3);
+        const uint32_t nrm_base = static_cast<uint32_t>(m.normals.size() / 4);
+        const uint32_t uv_base = static_cast<uint32_t>(m.uvs.size() / 2);
+        const uint32_t tex_base = static_cast<uint32_t>(m.textures.size());
+        if (uint64_t(m.tlas.size()) + d->tlas_nodes_len >= 0x7ffffff0u ||
+            uint64_t(m.blas.size()) + d->blas_nodes_len >= 0x7ffffff0u)
+            return ORN_ERROR_INVALID_SCENE;
+        for (size_t k = 0; k < d->tlas_nodes_len; k++) {
+            orn_bvh_node nd = d->tlas_nodes[k];
+            if (nd.node_type == 0u) {
+                nd.left_or_custom_id += tlas_base;
+                nd.right_or_material_index += tlas_base;
+            } else {
+                if (nd.node_type == 2u) nd.left_or_custom_id += blas_base;
+                nd.right_or_material_index += mat_base;
+                nd.transform_id += tid_base;
+            }
+            m.tlas.push_back(nd);
+        }
+        for (size_t k = 0; k < d->blas_nodes_len; k++) {
+            orn_bvh_node nd = d->blas_nodes[k];
+            if (nd.node_type == 0u) {
+                nd.left_or_custom_id += blas_base;
+                nd.right_or_material_index += blas_base;
+            } else {
+                nd.left_or_custom_id += tri_base;
+            }
+            m.blas.push_back(nd);
+        }
+        m.normals.insert(m.normals.end(), d->normals, d->normals + 4 * d->normals_len);
+        m.uvs.insert(m.uvs.end(), d->uvs, d->uvs + 2 * d->uvs_len);
+        // normal and uv indices share the triangle id (3 per triangle): both
+        // are padded to one length, a multiple of 3; padding is never read
+        const size_t len = (std::max(d->normal_indices_len, d->uv_indices_len) + 2) / 3 * 3;
+        for (size_t k = 0; k < len; k++) {
+            m.normal_indices.push_back(k < d->normal_indices_len ? d->normal_indices[k] + nrm_base : 0u);
+            m.uv_indices.push_back(k < d->uv_indices_len ? d->uv_indices[k] + uv_base : 0u);
+        }
+        m.transforms.insert(m.transforms.end(), d->transforms, d->transforms + 16 * d->transforms_len);
+        if (d->transforms_len % 2) m.transforms.insert(m.transforms.end(), 16, 0.0f);
+        for (size_t k = 0; k < d->materials_len; k++) {
+            orn_material mt = d->materials[k];
+            mt.albedo_texture_index = mt.albedo_texture_index < d->textures_len
+                                          ? mt.albedo_texture_index + tex_base
+                                          : 0xffffffffu;
+            m.materials.push_back(mt);
+        }
+        m.textures.insert(m.textures.end(), d->textures, d->textures + d->textures_len);
+        m.roots.push_back(static_cast<uint32_t>(tlas_base + d->tlas_nodes_len - 1));
+        m.tex_base.push_back(tex_base);
+    }
+    orn_scene_desc &o = m.desc;
+    o.tlas_nodes = m.tlas.data();
+    o.tlas_nodes_len = m.tlas.size();
+    o.blas_nodes = m.blas.data();
+    o.blas_nodes_len = m.blas.size();
+    o.normals = m.normals.data();
+    o.normals_len = m.normals.size() / 4;
+    o.normal_indices = m.normal_indices.data();
+    o.normal_indices_len = m.normal_indices.size();
+    o.uvs = m.uvs.data();
+    o.uvs_len = m.uvs.size() / 2;
+    o.uv_indices = m.uv_indices.data();
+    o.uv_indices_len = m.uv_indices.size();
+    o.transforms = m.transforms.data();
+    o.transforms_len = m.transforms.size() / 16;
+    o.materials = m.materials.data();
+    o.materials_len = m.materials.size();
+    o.textures = m.textures.data();
+    o.textures_len = m.textures.size();
+    return ORN_OK;
+}
+
+}  // namespace
+
+struct orn_batch {
+    orn_device_scene *scene;
+    size_t n;
+    std::vector<uint32_t> width, height, tiles_x, tiles, px_base, root2, root4, tex_base;
+    uint32_t total_px;
+    uint32_t total_tiles;
+    BatchItem *h_items;  // pinned staging of the item table, read by the graph's copy
+    BatchItem *d_items;
+    BatchTiles *h_work;  // pinned staging of the tile work list
+    BatchTiles *d_work;
+    size_t work_bytes;
+    std::vector<uint64_t> work_cost;  // per-image costs the current work list was ordered by
+    float4 *fb, *acc;
+    uint32_t *rng;
+    uint32_t *queues;
+    orn_tuning tuning;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool graph_ok;
+    hipEvent_t ev_start, ev_stop;
+    bool timed;
+};
+
+namespace {
+
+void free_batch(orn_batch *b)
+{
+    orn_device_scene *s = b->scene;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->h_items) (void)hipHostFree(b->h_items);
+    if (b->h_work) (void)hipHostFree(b->h_work);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_work) (void)hipFree(b->d_work);
+    if (b->fb) (void)hipFree(b->fb);
+    if (b->acc) (void)hipFree(b->acc);
+    if (b->rng) (void)hipFree(b->rng);
+    if (b->queues) (void)hipFree(b->queues);
+    if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+    if (b->ev_stop) (void)hipEventDestroy(b->ev_stop);
+    delete b;
+    release_scene(s);
+}
+
+// Orders the tile work list by the images' expected cost per pixel
+// (samples x (depth + 1)), most expensive first, and deals it out to the
+// eight bands in runs of 4 tiles, so every band starts on the long pixels
+// and all bands hold about the same work.
+void build_work(orn_batch *b, const std::vector<uint64_t> &cost)
+{
+    std::vector<uint32_t> order(b->n);
+    for (size_t i = 0; i < b->n; i++) order[i] = static_cast<uint32_t>(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
+    std::vector<std::vector<uint2>> band(kXcds);
+    constexpr uint32_t kRun = 4;
+    uint32_t dealt = 0;
+    for (uint32_t i : order)
+        for (uint32_t t = 0; t < b->tiles[i]; t++, dealt++) band[(dealt / kRun) % kXcds].push_back(make_uint2(i, t));
+    uint32_t at = 0;
+    for (uint32_t k = 0; k < kXcds; k++) {
+        b->h_work->band_begin[k] = at;
+        b->h_work->band_tiles[k] = static_cast<uint32_t>(band[k].size());
+        std::memcpy(&b->h_work->tiles[at], band[k].data(), band[k].size() * sizeof(uint2));
+        at += static_cast<uint32_t>(band[k].size());
+    }
+    b->work_cost = cost;
+}
+
+// Captures [item table copy, queue reset, render launch] as one graph.
+orn_status capture_batch(orn_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_ok = false;
+    orn_device_scene *s = b->scene;
+    hipStream_t st = s->ctx->stream;
+    const uint32_t width = b->tuning.bvh_width == 4u ? 4u : 2u;
+    DevScene ds = s->ds;
+    ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
+    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+    KernelParams kp{};
+    kp.regroup = b->tuning.regroup_lanes;
+    kp.leaf_regroup = b->tuning.leaf_regroup_lanes;
+    LaunchArgs a{kp, b->d_items, b->d_work, b->acc, b->rng, b->fb, b->queues, b->total_tiles};
+    ORN_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    hipError_t e = hipMemcpyAsync(b->d_items, b->h_items, b->n * sizeof(BatchItem), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        reset_queues_kernel<<<1, 64, 0, st>>>(b->queues);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = launch_render<true>(s->features, width, b->tuning.workgroups_per_cu,
+                                static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    b->graph = g;
+    if (e != hipSuccess) return from_hip(e);
+    b->graph_ok = true;
+    return ORN_OK;
+}
+
+orn_status batch_readback(orn_batch *b, size_t item, void *dst, const void *pool, size_t elem, size_t pixels)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (!dst || item >= b->n) return ORN_ERROR_INVALID_VALUE;
+    const size_t n = size_t(b->width[item]) * b->height[item];
+    if (pixels < n) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(b->scene->ctx->device));
+    hipStream_t st = b->scene->ctx->stream;
+    const char *src = static_cast<const char *>(pool) + size_t(b->px_base[item]) * elem;
+    ORN_TRY(hipMemcpyAsync(dst, src, n * elem, hipMemcpyDeviceToHost, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+orn_status orn_batch_create(orn_ctx *ctx, const orn_batch_item *items, size_t n, orn_batch **out)
+{
+    if (!ctx) return ORN_ERROR_INVALID_HANDLE;
+    if (!out || !items || n == 0 || n > 0xffffffu) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    uint64_t px = 0, tiles = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (items[i].width == 0 || items[i].height == 0) return ORN_ERROR_INVALID_VALUE;
+        px += uint64_t(items[i].width) * items[i].height;
+        tiles += uint64_t(blocks_for(items[i].width, kTile)) * blocks_for(items[i].height, kTile);
+    }
+    if (px > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
+    MergedScene m;
+    uint32_t slots = 0;
+    orn_status st = merge_scenes(items, n, m, &slots);
+    if (st != ORN_OK) return st;
+    std::vector<uint32_t> root2(n), root4(n);
+    orn_device_scene *s = nullptr;
+    st = upload_scene(ctx, &m.desc, slots, m.roots.data(), n, root2.data(), root4.data(), &s);
+    if (st != ORN_OK) return st;
+    orn_batch *b = new (std::nothrow) orn_batch();
+    if (!b) {
+        release_scene(s);
+        return ORN_ERROR_OUT_OF_MEMORY;
+    }
+    b->scene = s;  // the batch holds the scene's only reference
+    b->n = n;
+    b->root2 = root2;
+    b->root4 = root4;
+    b->tex_base = m.tex_base;
+    b->tuning = default_tuning();
+    uint32_t base = 0;
+    for (size_t i = 0; i < n; i++) {
+        b->width.push_back(items[i].width);
+        b->height.push_back(items[i].height);
+        b->tiles_x.push_back(blocks_for(items[i].width, kTile));
+        b->tiles.push_back(b->tiles_x.back() * blocks_for(items[i].height, kTile));
+        b->px_base.push_back(base);
+        base += items[i].width * items[i].height;
+    }
+    b->total_px = base;
+    b->total_tiles = static_cast<uint32_t>(tiles);
+    b->work_bytes = sizeof(BatchTiles) + tiles * sizeof(uint2);
+    auto fail = [&](hipError_t e) {
+        free_batch(b);
+        return from_hip(e);
+    };
+    hipError_t e;
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&b->h_items), n * sizeof(BatchItem), hipHostMallocDefault)) !=
+        hipSuccess)
+        return fail(e);
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&b->h_work), b->work_bytes, hipHostMallocDefault)) != hipSuccess)
+        return fail(e);
+    if ((e = hipMalloc(&b->d_items, n * sizeof(BatchItem))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&b->d_work, b->work_bytes)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&b->fb, size_t(base) * sizeof(float4))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&b->acc, size_t(base) * sizeof(float4))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&b->rng, size_t(base) * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&b->queues, 64 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipEventCreate(&b->ev_start)) != hipSuccess) return fail(e);
+    if ((e = hipEventCreate(&b->ev_stop)) != hipSuccess) return fail(e);
+    std::memset(b->h_items, 0, n * sizeof(BatchItem));
+    hipStream_t stream = ctx->stream;
+    zero_kernel<<<blocks_for(base, 256), 256, 0, stream>>>(b->fb, b->acc, base);
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t nn = items[i].width * items[i].height;
+        seed_rng_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(b->rng + b->px_base[i], nn, items[i].seed * nn);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e);
+    *out = b;
+    return ORN_OK;
+}
+
+orn_status orn_batch_destroy(orn_batch *b)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    free_batch(b);
+    return ORN_OK;
+}
+
+orn_status orn_batch_render_async(orn_batch *b, const orn_constant_params *params, const uint32_t *iterations)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (!params || !iterations) return ORN_ERROR_INVALID_VALUE;
+    for (size_t i = 0; i < b->n; i++)
+        if (params[i].width != b->width[i] || params[i].height != b->height[i]) return ORN_ERROR_SIZE_MISMATCH;
+    orn_device_scene *s = b->scene;
+    ORN_TRY(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    // the previous launch may still read the pinned staging buffers
+    ORN_TRY(hipStreamSynchronize(st));
+    const bool w4 = b->tuning.bvh_width == 4u;
+    std::vector<uint64_t> cost(b->n);
+    for (size_t i = 0; i < b->n; i++) {
+        BatchItem &it = b->h_items[i];
+        it.p = params[i];
+        it.iterations = iterations[i];
+        it.tiles_x = b->tiles_x[i];
+        it.px_base = b->px_base[i];
+        it.root = w4 ? b->root4[i] : b->root2[i];
+        it.tex_base = b->tex_base[i];
+        cost[i] = uint64_t(std::max(iterations[i], 1u)) * (uint64_t(params[i].depth) + 1u);
+    }
+    if (cost != b->work_cost) {
+        build_work(b, cost);
+        ORN_TRY(hipMemcpyAsync(b->d_work, b->h_work, b->work_bytes, hipMemcpyHostToDevice, st));
+    }
+    if (!b->graph_ok) {
+        const orn_status cs = capture_batch(b);
+        if (cs != ORN_OK) return cs;
+    }
+    ORN_TRY(hipEventRecord(b->ev_start, st));
+    ORN_TRY(hipGraphLaunch(b->exec, st));
+    ORN_TRY(hipEventRecord(b->ev_stop, st));
+    b->timed = true;
+    return ORN_OK;
+}
+
+orn_status orn_batch_wait(orn_batch *b)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    ORN_TRY(hipSetDevice(b->scene->ctx->device));
+    ORN_TRY(hipStreamSynchronize(b->scene->ctx->stream));
+    return ORN_OK;
+}
+
+orn_status orn_batch_render(orn_batch *b, const orn_constant_params *params, const uint32_t *iterations)
+{
+    const orn_status s = orn_batch_render_async(b, params, iterations);
+    if (s != ORN_OK) return s;
+    return orn_batch_wait(b);
+}
+
+orn_status orn_batch_set_tuning(orn_batch *b, const orn_tuning *tuning)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64 ||
+        (tuning->bvh_width != 2u && tuning->bvh_width != 4u) || tuning->leaf_regroup_lanes < 1 ||
+        tuning->leaf_regroup_lanes > 64)
+        return ORN_ERROR_INVALID_VALUE;
+    if (std::memcmp(tuning, &b->tuning, sizeof(orn_tuning)) != 0) {
+        ORN_TRY(hipSetDevice(b->scene->ctx->device));
+        ORN_TRY(hipStreamSynchronize(b->scene->ctx->stream));  // the graph may be running
+        b->tuning = *tuning;
+        b->graph_ok = false;  // captured again at the next render
+    }
+    return ORN_OK;
+}
+
+orn_status orn_batch_get_tuning(orn_batch *b, orn_tuning *tuning)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (!tuning) return ORN_ERROR_INVALID_VALUE;
+    *tuning = b->tuning;
+    return ORN_OK;
+}
+
+orn_status orn_batch_last_render_ms(orn_batch *b, float *ms)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (!ms) return ORN_ERROR_INVALID_VALUE;
+    *ms = 0.0f;
+    if (!b->timed) return ORN_OK;
+    ORN_TRY(hipEventSynchronize(b->ev_stop));
+    ORN_TRY(hipEventElapsedTime(ms, b->ev_start, b->ev_stop));
+    return ORN_OK;
+}
+
+orn_status orn_batch_get_framebuffer(orn_batch *b, size_t item, float *dst, size_t pixels)
+{
+    return batch_readback(b, item, dst, b ? b->fb : nullptr, sizeof(float4), pixels);
+}
+
+orn_status orn_batch_get_accumulation(orn_batch *b, size_t item, float *dst, size_t pixels)
+{
+    return batch_readback(b, item, dst, b ? b->acc : nullptr, sizeof(float4), pixels);
+}
+
+orn_status orn_batch_get_rng_state(orn_batch *b, size_t item, uint32_t *dst, size_t pixels)
+{
+    return batch_readback(b, item, dst, b ? b->rng : nullptr, sizeof(uint32_t), pixels);
+}
+
+orn_status orn_batch_set_state(orn_batch *b, size_t item, const float *accumulation, const uint32_t *rng,
+                               size_t pixels)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (item >= b->n) return ORN_ERROR_INVALID_VALUE;
+    const size_t n = size_t(b->width[item]) * b->height[item];
+    if (pixels != n) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(b->scene->ctx->device));
+    hipStream_t st = b->scene->ctx->stream;
+    const size_t at = b->px_base[item];
+    if (accumulation)
+        ORN_TRY(hipMemcpyAsync(b->acc + at, accumulation, n * sizeof(float4), hipMemcpyHostToDevice, st));
+    if (rng) ORN_TRY(hipMemcpyAsync(b->rng + at, rng, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Autotuning
+// ===========================================================================
+
+namespace {
+
+// Flat JSON cache: {"<key>": {"<field>": <number>, ...}, ...}
+using TuneCache = std::vector<std::pair<std::string, std::vector<std::pair<std::string, double>>>>;
+
+void skip_ws(const std::string &t, size_t &i)
+{
+    while (i < t.size() && (t[i] == ' ' || t[i] == '\n' || t[i] == '\r' || t[i] == '\t')) i++;
+}
+
+bool parse_string(const std::string &t, size_t &i, std::string &out)
+{
+    skip_ws(t, i);
+    if (i >= t.size() || t[i] != '"') return false;
+    const size_t end = t.find('"', i + 1);
+    if (end == std::string::npos) return false;
+    out = t.substr(i + 1, end - i - 1);
+    i = end + 1;
+    return true;
+}
+
+bool expect(const std::string &t, size_t &i, char c)
+{
+    skip_ws(t, i);
+    if (i >= t.size() || t[i] != c) return false;
+    i++;
+    return true;
+}
+
+// Reads the cache; a missing or malformed file reads as empty.
+TuneCache load_cache(const char *path)
+{
+    TuneCache c;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return c;
+    std::string t;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) t.append(buf, n);
+    std::fclose(f);
+    size_t i = 0;
+    if (!expect(t, i, '{')) return {};
+    skip_ws(t, i);
+    if (i < t.size() && t[i] == '}') return c;
+    for (;;) {
+        std::string key;
+        if (!parse_string(t, i, key) || !expect(t, i, ':') || !expect(t, i, '{')) return {};
+        std::vector<std::pair<std::string, double>> fields;
+        skip_ws(t, i);
+        if (i < t.size() && t[i] == '}') {
+            i++;
+        } else {
+            for (;;) {
+                std::string name;
+                if (!parse_string(t, i, name) || !expect(t, i, ':')) return {};
+                skip_ws(t, i);
+                char *endp = nullptr;
+                const double v = std::strtod(t.c_str() + i, &endp);
+                if (endp == t.c_str() + i) return {};
+                i = static_cast<size_t>(endp - t.c_str());
+                fields.emplace_back(name, v);
+                skip_ws(t, i);
+                if (i < t.size() && t[i] == ',') {
+                    i++;
+                    continue;
+                }
+                if (!expect(t, i, '}')) return {};
+                break;
+            }
+        }
+        c.emplace_back(key, fields);
+        skip_ws(t, i);
+        if (i < t.size() && t[i] == ',') {
+            i++;
+            continue;
+        }
+        if (!expect(t, i, '}')) return {};
+        return c;
+    }
+}
+
+bool save_cache(const char *path, const TuneCache &c)
+{
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return false;
+    std::fprintf(f, "{\n");
+    for (size_t k = 0; k < c.size(); k++) {
+        std::fprintf(f, "  \"%s\": {", c[k].first.c_str());
+        for (size_t j = 0; j < c[k].second.size(); j++)
+            std::fprintf(f, "%s\"%s\": %.9g", j ? ", " : "", c[k].second[j].first.c_str(), c[k].second[j].second);
+        std::fprintf(f, "}%s\n", k + 1 < c.size() ? "," : "");
+    }
+    std::fprintf(f, "}\n");
+    const bool ok = std::fclose(f) == 0;
+    return ok && std::rename(tmp.c_str(), path) == 0;
+}
+
+uint32_t log2_bucket(uint64_t v)
+{
+    uint32_t b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+// Scene class: what decides the best launch shape, bucketed so that
+// similar scenes share one entry.
+std::string scene_class(const orn_device_scene *s, uint64_t pixels, uint32_t spp, uint32_t depth, size_t images,
+                        uint32_t flags)
+{
+    char key[256];
+    std::snprintf(key, sizeof(key), "%s|features%u|tlas2^%u|blas2^%u|px2^%u|spp2^%u|depth%u|images2^%u|bvh4%u",
+                  s->ctx->arch, s->features, log2_bucket(s->ds.tlas_len), log2_bucket(s->blas_len),
+                  log2_bucket(pixels), log2_bucket(std::max(spp, 1u)), depth, log2_bucket(images),
+                  (flags & ORN_TUNE_ALLOW_BVH4) ? 1u : 0u);
+    return key;
+}
+
+bool cached_tuning(const char *path, const std::string &key, orn_tuning *t, float *ms)
+{
+    if (!path) return false;
+    const TuneCache c = load_cache(path);
+    for (const auto &e : c) {
+        if (e.first != key) continue;
+        orn_tuning r{};
+        bool ok[4] = {false, false, false, false};
+        for (const auto &f : e.second) {
+            const uint32_t v = static_cast<uint32_t>(f.second);
+            if (f.first == "regroup_lanes") r.regroup_lanes = v, ok[0] = true;
+            else if (f.first == "leaf_regroup_lanes") r.leaf_regroup_lanes = v, ok[1] = true;
+            else if (f.first == "workgroups_per_cu") r.workgroups_per_cu = v, ok[2] = true;
+            else if (f.first == "bvh_width") r.bvh_width = v, ok[3] = true;
+            else if (f.first == "ms") *ms = static_cast<float>(f.second);
+        }
+        if (!(ok[0] && ok[1] && ok[2] && ok[3])) return false;
+        if (r.regroup_lanes < 1 || r.regroup_lanes > 64 || r.leaf_regroup_lanes < 1 || r.leaf_regroup_lanes > 64 ||
+            r.workgroups_per_cu > 64 || (r.bvh_width != 2u && r.bvh_width != 4u))
+            return false;
+        *t = r;
+        return true;
+    }
+    return false;
+}
+
+void store_tuning(const char *path, const std::string &key, const orn_tuning &t, float ms)
+{
+    if (!path) return;
+    TuneCache c = load_cache(path);
+    std::vector<std::pair<std::string, double>> fields = {
+        {"regroup_lanes", t.regroup_lanes},
+        {"leaf_regroup_lanes", t.leaf_regroup_lanes},
+        {"workgroups_per_cu", t.workgroups_per_cu},
+        {"bvh_width", t.bvh_width},
+        {"ms", ms}};
+    bool found = false;
+    for (auto &e : c)
+        if (e.first == key) {
+            e.second = fields;
+            found = true;
+        }
+    if (!found) c.emplace_back(key, fields);
+    (void)save_cache(path, c);
+}
+
+// Coordinate descent over the tuning fields; `run` renders once under a
+// tuning and returns its GPU time. Each candidate keeps the faster of two
+// renders.
+template <class Run>
+orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, orn_tuning *best, float *best_ms)
+{
+    std::vector<std::pair<orn_tuning, float>> seen;
+    auto measure = [&](const orn_tuning &t, float *ms) -> orn_status {
+        for (const auto &e : seen)
+            if (std::memcmp(&e.first, &t, sizeof(orn_tuning)) == 0) {
+                *ms = e.second;
+                return ORN_OK;
+            }
+        float a = 0, b = 0;
+        orn_status st = run(t, &a);
+        if (st == ORN_OK) st = run(t, &b);
+        if (st != ORN_OK) return st;
+        *ms = std::min(a, b);
+        seen.emplace_back(t, *ms);
+        return ORN_OK;
+    };
+    orn_tuning cur = start;
+    float cur_ms = 0;
+    orn_status st = measure(cur, &cur_ms);
+    if (st != ORN_OK) return st;
+    static const uint32_t regroup[] = {8, 16, 24, 32, 48, 64};
+    static const uint32_t leaf[] = {1, 4, 8, 16, 32};
+    static const uint32_t per_cu[] = {0, 1, 2, 3};
+    static const uint32_t width[] = {2, 4};
+    auto sweep = [&](uint32_t orn_tuning::*field, const uint32_t *vals, size_t n) -> orn_status {
+        for (size_t i = 0; i < n; i++) {
+            orn_tuning t = cur;
+            t.*field = vals[i];
+            float ms = 0;
+            const orn_status s = measure(t, &ms);
+            if (s != ORN_OK) return s;
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+        return ORN_OK;
+    };
+    if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
+    if ((st = sweep(&orn_tuning::leaf_regroup_lanes, leaf, 5)) != ORN_OK) return st;
+    if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
+    if ((st = sweep(&orn_tuning::workgroups_per_cu, per_cu, 4)) != ORN_OK) return st;
+    if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
+    *best = cur;
+    *best_ms = cur_ms;
+    return ORN_OK;
+}
+
+// Device copies of a target's or batch's pixel state, restored after
+// every trial render.
+struct SavedState {
+    void *acc = nullptr, *rng = nullptr, *fb = nullptr;
+    size_t px = 0;
+    hipStream_t st = nullptr;
+    ~SavedState()
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        if (acc) (void)hipFree(acc);
+        if (rng) (void)hipFree(rng);
+        if (fb) (void)hipFree(fb);
+    }
+    hipError_t save(hipStream_t s, const float4 *a, const uint32_t *r, const float4 *f, size_t n)
+    {
+        st = s;
+        px = n;
+        hipError_t e = hipMalloc(&acc, n * sizeof(float4));
+        if (e == hipSuccess) e = hipMalloc(&rng, n * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMalloc(&fb, n * sizeof(float4));
+        if (e == hipSuccess) e = hipMemcpyAsync(acc, a, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(rng, r, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(fb, f, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
+        return e;
+    }
+    hipError_t restore(float4 *a, uint32_t *r, float4 *f) const
+    {
+        hipError_t e = hipMemcpyAsync(a, acc, px * sizeof(float4), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(r, rng, px * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(f, fb, px * sizeof(float4), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        return e;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+orn_status orn_target_autotune(orn_target *t, const orn_constant_params *params, uint32_t iterations, uint32_t flags,
+                               const char *cache_path, orn_tuning *chosen, float *ms)
+{
+    if (!t) return ORN_ERROR_INVALID_HANDLE;
+    if (!params) return ORN_ERROR_INVALID_VALUE;
+    if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
+    orn_device_scene *s = t->scene;
+    const std::string key =
+        scene_class(s, uint64_t(t->width) * t->height, iterations, params->depth, 1, flags);
+    orn_tuning best{};
+    float best_ms = 0.0f;
+    if (!(flags & ORN_TUNE_IGNORE_CACHE) && cached_tuning(cache_path, key, &best, &best_ms)) {
+        t->tuning = best;
+    } else {
+        ORN_TRY(hipSetDevice(s->ctx->device));
+        hipStream_t st = s->ctx->stream;
+        ORN_TRY(hipStreamSynchronize(st));
+        SavedState saved;
+        ORN_TRY(saved.save(st, t->acc, t->rng, t->fb, size_t(t->width) * t->height));
+        const orn_tuning start = t->tuning;
+        auto run = [&](const orn_tuning &tn, float *m) -> orn_status {
+            t->tuning = tn;
+            orn_status r = orn_render(t, params, iterations);
+            if (r == ORN_OK) r = orn_target_last_render_ms(t, m);
+            const hipError_t e = saved.restore(t->acc, t->rng, t->fb);
+            return r != ORN_OK ? r : from_hip(e);
+        };
+        const orn_status r = coordinate_descent(start, flags, run, &best, &best_ms);
+        t->tuning = r == ORN_OK ? best : start;
+        if (r != ORN_OK) return r;
+        store_tuning(cache_path, key, best, best_ms);
+    }
+    if (chosen) *chosen = best;
+    if (ms) *ms = best_ms;
+    return ORN_OK;
+}
+
+orn_status orn_batch_autotune(orn_batch *b, const orn_constant_params *params, const uint32_t *iterations,
+                              uint32_t flags, const char *cache_path, orn_tuning *chosen, float *ms)
+{
+    if (!b) return ORN_ERROR_INVALID_HANDLE;
+    if (!params || !iterations) return ORN_ERROR_INVALID_VALUE;
+    uint32_t spp = 0, depth = 0;
+    for (size_t i = 0; i < b->n; i++) {
+        if (params[i].width != b->width[i] || params[i].height != b->height[i]) return ORN_ERROR_SIZE_MISMATCH;
+        spp = std::max(spp, iterations[i]);
+        depth = std::max(depth, params[i].depth);
+    }
+    orn_device_scene *s = b->scene;
+    const std::string key = scene_class(s, b->total_px, spp, depth, b->n, flags);
+    orn_tuning best{};
+    float best_ms = 0.0f;
+    if (!(flags & ORN_TUNE_IGNORE_CACHE) && cached_tuning(cache_path, key, &best, &best_ms)) {
+        const orn_status r = orn_batch_set_tuning(b, &best);
+        if (r != ORN_OK) return r;
+    } else {
+        ORN_TRY(hipSetDevice(s->ctx->device));
+        hipStream_t st = s->ctx->stream;
+        ORN_TRY(hipStreamSynchronize(st));
+        SavedState saved;
+        ORN_TRY(saved.save(st, b->acc, b->rng, b->fb, b->total_px));
+        const orn_tuning start = b->tuning;
+        auto run = [&](const orn_tuning &tn, float *m) -> orn_status {
+            orn_status r = orn_batch_set_tuning(b, &tn);
+            if (r == ORN_OK) r = orn_batch_render(b, params, iterations);
+            if (r == ORN_OK) r = orn_batch_last_render_ms(b, m);
+            const hipError_t e = saved.restore(b->acc, b->rng, b->fb);
+            return r != ORN_OK ? r : from_hip(e);
+        };
+        const orn_status r = coordinate_descent(start, flags, run, &best, &best_ms);
+        (void)orn_batch_set_tuning(b, r == ORN_OK ? &best : &start);
+        if (r != ORN_OK) return r;
+        store_tuning(cache_path, key, best, best_ms);
+    }
+    if (chosen) *chosen = best;
+    if (ms) *ms = best_ms;
+    return ORN_OK;
 }
 
 }  // extern "C"

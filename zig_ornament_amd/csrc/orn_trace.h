@@ -193,14 +193,14 @@ struct Traversal {
     Hit h;
 };
 
-template <uint32_t W>
-__device__ __forceinline__ void trav_begin(const DevScene &sc, const Ray &world, Traversal &tr)
+// `root`: the TLAS root reference in the traversed format (BVH2 or BVH4).
+__device__ __forceinline__ void trav_begin(uint32_t root, const Ray &world, Traversal &tr)
 {
     tr.ray = world;
     tr.inv = safe_invdir(world.d);
     tr.ox = (-world.o) * tr.inv;
     tr.t_max = kTMaxInit;
-    tr.addr = W == 4u ? sc.tlas4_root : sc.tlas2_root;
+    tr.addr = root;
     tr.top = 0u;
     tr.material_index = 0u;
     tr.inv_tid = 0u;
@@ -372,7 +372,7 @@ __device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const R
                                         uint32_t *stack, uint32_t stride, Hit &h)
 {
     Traversal tr;
-    trav_begin<W>(sc, world, tr);
+    trav_begin(W == 4u ? sc.tlas4_root : sc.tlas2_root, world, tr);
     while (trav_step<F, W>(sc, t_min, world, stack, stride, tr)) {
     }
     h = tr.h;
@@ -452,16 +452,6 @@ __device__ __forceinline__ f3 sample_texture(const DevScene &sc, uint32_t id, fl
     return f3{c[0], c[1], c[2]};
 }
 
-// material.wgsl:91-99. Textures beyond what was uploaded read as "no texture"
-// (the reference would index its binding array out of range there).
-__device__ __forceinline__ f3 material_color(const DevScene &sc, const Material &m,
-                                             uint32_t textures_count, float u, float v)
-{
-    if (m.texture < textures_count && m.texture < sc.textures_len)
-        return sample_texture(sc, m.texture, u, v);
-    return m.albedo;
-}
-
 __device__ __forceinline__ bool near_zero(f3 e)
 {
     return fabsf(e.x) < 1e-8f && fabsf(e.y) < 1e-8f && fabsf(e.z) < 1e-8f;
@@ -481,14 +471,21 @@ __device__ __forceinline__ f3 refract_dir(f3 e1, f3 e2, float eta)
 // Shading of one closest hit: fills the next ray and the attenuation, or the
 // emitted radiance when the path ends (pathtracer.wgsl:114-157).
 // Returns true when the path continues.
+// Texture lookup (material.wgsl:91-99): a material's texture is sampled when
+// its index, counted from `tex_base`, is below `textures_count`. tex_base is
+// the scene's first texture in the shared texture table (non-zero when
+// several scenes share one device scene; their materials then hold indices
+// into the shared table, and 0xffffffff for "none"). Textures beyond what was
+// uploaded read as "no texture" (the reference would index its binding array
+// out of range there).
 template <uint32_t F>
-__device__ __forceinline__ bool shade(const DevScene &sc, uint32_t textures_count, const Ray &ray,
-                                      const Hit &h, uint32_t &rng, Ray &next, f3 &weight)
+__device__ __forceinline__ bool shade(const DevScene &sc, uint32_t textures_count, uint32_t tex_base,
+                                      const Ray &ray, const Hit &h, uint32_t &rng, Ray &next, f3 &weight)
 {
     const f3 p = ray.o + h.t * ray.d;
     const Material m = load_material(sc, h.material);
     const bool textured =
-        (F & kHasTextures) && m.texture < textures_count && m.texture < sc.textures_len;
+        (F & kHasTextures) && m.texture - tex_base < textures_count && m.texture < sc.textures_len;
     f3 normal;
     bool front;
     float tu = 0.0f, tv = 0.0f;

@@ -14,6 +14,8 @@ Random choices use SplitMix64 (seeded), so every scene is reproducible:
     C3  a 7-level icosphere (327,680 triangles) instanced 1,024 times with
         random rotation, scale U(0.5, 2) and translation in a 100^3 box,
         textured by 1024^2 RGBA8 Perlin and checker textures; 1920x1080, 32 spp, depth 8
+    C4  a batch of 64 instanced-icosphere scenes, 1e3..2e6 triangles
+        (log-uniform), 128^2..1024^2 pixels, 1..64 spp, depth 8
 """
 import dataclasses
 import math
@@ -285,6 +287,59 @@ def instanced_icospheres(width=1920, height=1080, spp=32, depth=8, levels=7, ins
         m = mat_mul(mat_mul(mat_scaling(sc, sc, sc), mat_from_quat(q)), mat_translation(*tr))
         spec.instances.append((0, m, k % 2))
     return spec
+
+
+def instanced_scene(triangles, side, spp, depth=8, seed=5, texture_size=128):
+    """One image of the batch workload: icospheres instanced until the scene
+    holds about `triangles` triangles (level 3, 1,280 triangles each, below
+    20,480; level 4, 5,120 each, from there), in a box whose volume grows with
+    the instance count; textured Lambertian, metal and dielectric instances.
+    The base mesh is a TLAS leaf too (bvh.zig:101-104): it sits at the
+    origin at unit scale, inside the instance cloud."""
+    rng = SplitMix64(seed)
+    levels = 3 if triangles < 20480 else 4
+    per = 20 * 4 ** levels
+    instances = max(1, int(round(triangles / per)))
+    box = 6.0 * instances ** (1.0 / 3.0)
+    spec = SceneSpec(f"instanced_{triangles}_{side}px_{spp}spp", side, side, spp, depth,
+                     camera=((0.0, 0.4 * box, -1.5 * box), (0.0, 0.0, 0.0), (0, 1, 0), 1.0,
+                             40.0, 0.0, 10.0))
+    spec.textures = [
+        dict(data=perlin_texture(texture_size, seed + 1), width=texture_size,
+             height=texture_size, num_components=4, bytes_per_component=1, is_hdr=False),
+        dict(data=checker_texture(texture_size), width=texture_size, height=texture_size,
+             num_components=4, bytes_per_component=1, is_hdr=False),
+    ]
+    spec.materials = [("lambertian", ("tex", 0), None), ("lambertian", ("tex", 1), None),
+                      ("metal", (0.8, 0.7, 0.6), 0.2), ("dielectric", None, 1.5)]
+    verts, vi, normals, ni, uvs, ui = icosphere_mesh(levels)
+    spec.meshes.append(("mesh", verts, vi, normals, ni, uvs, ui, np.eye(4, dtype=np.float32), 0))
+    for k in range(instances):
+        u1, u2, u3 = rng.float(), rng.float(), rng.float()
+        q = (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+             math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+        sc = 0.5 + 1.5 * rng.float()
+        tr = [box * (rng.float() - 0.5) for _ in range(3)]
+        m = mat_mul(mat_mul(mat_scaling(sc, sc, sc), mat_from_quat(q)), mat_translation(*tr))
+        spec.instances.append((0, m, k % 4))
+    return spec
+
+
+def batch_scenes(count=64, seed=4, min_tris=1e3, max_tris=2e6, sides=(128, 256, 512, 1024),
+                 spps=(1, 2, 4, 8, 16, 32, 64), depth=8):
+    """C4: `count` images, triangle counts log-uniform in [min_tris, max_tris]
+    (instanced icospheres), square resolutions drawn from `sides`, samples
+    per pixel drawn from `spps`, each with its own RNG seed."""
+    rng = SplitMix64(seed)
+    specs = []
+    for i in range(count):
+        tris = int(round(math.exp(math.log(min_tris) + rng.float() * math.log(max_tris / min_tris))))
+        side = sides[int(rng.float() * len(sides))]
+        spp = spps[int(rng.float() * len(spps))]
+        spec = instanced_scene(tris, side, spp, depth, seed=1000 + i)
+        spec.seed = i
+        specs.append(spec)
+    return specs
 
 
 CONFIGS = {
