@@ -6,20 +6,28 @@ seed 0. One step = one PathTracer.render() of 64 spp over the whole image
 (wgpu_backend/path_tracer.zig:252-265), i.e. 132,710,400 samples, with the
 scene already resident in HBM. Metric: samples/s = W*H*spp*steps / wall time.
 
+Launch tuning: the library's autotuner picks the render kernel's launch
+parameters for the scene class; its JSON cache ships with the package
+(zig_ornament_amd/tuning_cache.json), so a known class is applied without
+trial renders. Tuning never changes the image.
+
 Also reported:
   roofline     algorithmic bytes per render (the reference algorithm's node,
                transform, material and attribute fetches, counted on the same
                paths by the instrumented CPU oracle in the cpu_baseline leg)
                divided by the render kernel's average duration (HIP events on
-               the stream the kernel runs on), against 8 TB/s of HBM3E.
+               the stream the kernel runs on), against 8 TB/s of HBM3E;
+               `traffic` is the HBM bytes per launch that rocprofv3's
+               FETCH_SIZE and WRITE_SIZE counters measured for this command,
+               read from the committed profile summary.
   cpu_baseline the oracle (a C restatement of the WGSL megakernel, OpenMP
                over pixels, serial samples per pixel) timed on the host on a
                bounded sample of the same workload.
 """
 import argparse
-import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,6 +40,18 @@ import zig_ornament_amd as zo  # noqa: E402
 from zig_ornament_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "render_kernel_random_spheres_1080p_64spp_pmc.md")
+
+
+def profiled_traffic():
+    """HBM bytes per render launch from the committed rocprofv3 counter
+    summary (`traffic_bytes_per_launch = N`), or None."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9]+)", text)
+    return int(m.group(1)) if m else None
 
 
 def algorithmic_bytes(stats):
@@ -84,6 +104,9 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel of this size (for scaling profiler "
                          "traffic counters); reported as calibration_copy")
+    ap.add_argument("--tuning-cache", default=zo.TUNING_CACHE,
+                    help="autotuner cache (JSON); a scene class missing from it is measured")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default launch tuning")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path tracer shards by pixel tiles with no "
@@ -107,6 +130,11 @@ def main():
     params.ray_cast_epsilon = spec.ray_cast_epsilon
     params.textures_count = len(spec.textures)
     params.current_iteration = 0.0
+
+    tuning_ms = None
+    if not args.no_autotune:
+        _, tuning_ms = tgt.autotune(params, spec.spp, cache_path=args.tuning_cache)
+    tuning = tgt.tuning()
 
     calib = None
     if args.calibration_copy_mib:
@@ -145,7 +173,8 @@ def main():
                    "height": spec.height, "spp": spec.spp, "depth": spec.depth,
                    "spheres": len(spec.spheres), "tlas_nodes": int(len(arrays.tlas_nodes)),
                    "parallelism": "single GPU, pixel-parallel", "device": name,
-                   "compute_units": cus, "bvh_build_s": build_s},
+                   "compute_units": cus, "bvh_build_s": build_s, "launch_tuning": tuning,
+                   "autotuner_ms": tuning_ms},
         "kernel_ms_avg": avg_kernel_s * 1e3,
     }
     if calib:
@@ -155,7 +184,8 @@ def main():
         bytes_per_launch = cb["bytes_per_sample"] * spec.samples + 16 * spec.width * spec.height
         achieved = bytes_per_launch / avg_kernel_s / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": profiled_traffic(),
                            "algorithmic_bytes_per_sample": cb["bytes_per_sample"],
                            "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
                            "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"]}

@@ -202,7 +202,8 @@ orn_status orn_scene_set_trace_width(orn_device_scene *scene, uint32_t width);
 
 /* Framebuffer and accumulation (float4 per pixel, zero-initialised) and the
  * per-pixel RNG state, seeded index + seed * width * height (seed 0 is the
- * reference's seeding, buffers.zig:25-30). */
+ * reference's seeding, buffers.zig:25-30). At most 2^28 pixels per target
+ * (and per batch, all images together). */
 orn_status orn_target_create(orn_device_scene *scene, uint32_t width, uint32_t height,
                              uint32_t seed, orn_target **out);
 orn_status orn_target_destroy(orn_target *target);
@@ -229,13 +230,21 @@ orn_status orn_target_wait(orn_target *target);
  *   of 1,024 instanced icospheres differ, by < 2e-7 relative).
  * leaf_regroup_lanes (1..64): lanes standing at a BVH leaf (sphere, triangle,
  *   mesh entry) wait until this many of the wave's lanes do, then test their
- *   leaves together (default 16). */
+ *   leaves together (default 16).
+ * split_percent (0..100), samples_per_job (0..65535): the last split_percent
+ *   percent of each XCD's pixels are rendered as jobs of samples_per_job
+ *   consecutive samples, handed from lane to lane through memory, so that no
+ *   lane is left with a whole pixel's samples at the end of a render (default
+ *   0 and 16; either 0: one job per pixel). Renders whose current_iteration
+ *   is not a whole number below 2^24 - iterations run one job per pixel. */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
     uint32_t bvh_width;
     uint32_t leaf_regroup_lanes;
-    uint32_t reserved[4];
+    uint32_t samples_per_job;
+    uint32_t split_percent;
+    uint32_t reserved[2];
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);
@@ -305,8 +314,9 @@ orn_status orn_batch_set_state(orn_batch *batch, size_t item, const float *accum
 
 /* Times renders of `iterations` samples per pixel (a few suffice) under
  * candidate tunings (coordinate descent over regroup_lanes,
- * leaf_regroup_lanes, workgroups_per_cu and, with ORN_TUNE_ALLOW_BVH4,
- * bvh_width), sets the fastest on the target and returns it with its time.
+ * leaf_regroup_lanes, workgroups_per_cu, split_percent, samples_per_job and,
+ * with ORN_TUNE_ALLOW_BVH4, bvh_width), sets the fastest on the target and
+ * returns it with its time.
  * The target's accumulation, RNG state and framebuffer are left as they
  * were. With a cache_path (a JSON file, created if absent), a tuning stored
  * for the same device architecture and scene class (scene features, BVH

@@ -76,8 +76,8 @@ def test_batch_matches_oracle_image(device, oracle):
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(regroup_lanes=8, leaf_regroup_lanes=1),
-    dict(regroup_lanes=64, leaf_regroup_lanes=32, workgroups_per_cu=1),
+    dict(regroup_lanes=8, leaf_regroup_lanes=1, samples_per_job=1, split_percent=100),
+    dict(regroup_lanes=64, leaf_regroup_lanes=32, workgroups_per_cu=1, samples_per_job=0),
 ])
 def test_batch_tunings_give_the_same_images(device, tuning):
     specs = mixed_specs()[:4]

@@ -45,7 +45,7 @@ def test_full_size_render(device, name):
     assert bad.mean() < 1e-5
     # determinism and progressive equivalence: spp/2 + spp/2 == spp
     b = zo.Target(ds, spec.width, spec.height)
-    b.set_tuning(regroup_lanes=16, bvh_width=2)
+    b.set_tuning(regroup_lanes=16, bvh_width=2, samples_per_job=0)
     b.render(zo_params(spec), spec.spp // 2)
     b.render(zo_params(spec, current_iteration=spec.spp // 2), spec.spp // 2)
     assert same_bits(b.accumulation(), acc_a)

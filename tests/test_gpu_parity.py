@@ -81,11 +81,14 @@ def test_trace_rays_match_fp64_brute_force(device, name):
 
 @pytest.mark.parametrize("tuning", [dict(bvh_width=4), dict(bvh_width=2),
                                     dict(bvh_width=4, regroup_lanes=1, workgroups_per_cu=1),
-                                    dict(bvh_width=2, regroup_lanes=64, workgroups_per_cu=3)])
+                                    dict(bvh_width=2, regroup_lanes=64, workgroups_per_cu=3),
+                                    dict(samples_per_job=1, split_percent=100, regroup_lanes=8),
+                                    dict(samples_per_job=0),
+                                    dict(samples_per_job=3, split_percent=60, workgroups_per_cu=1)])
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
 def test_image_matches_oracle(device, name, tuning):
-    """Any launch tuning (node width, regrouping, persistent grid size)
-    renders the oracle's image bit for bit."""
+    """Any launch tuning (node width, regrouping, persistent grid size, job
+    size) renders the oracle's image bit for bit."""
     spec = small_specs()[name]
     prod, _ = both_arrays(spec)
     n = spec.width * spec.height

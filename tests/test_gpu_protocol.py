@@ -132,3 +132,47 @@ def test_device_errors(device):
     with pytest.raises(zo.OrnamentError):
         zo.Target(ds, 0, 16)
     assert ds.trace(np.zeros((0, 6), np.float32)).shape == (0,)
+
+
+def test_fractional_iteration_counter_matches_oracle(device):
+    """A current_iteration that is not a whole number (never produced by
+    State, but accepted by the ABI) keeps the reference's per-sample f32
+    increments: such renders run one job per pixel."""
+    from helpers import zo_params
+    spec = W.cornell(width=24, height=24, spp=5)
+    prod = zo.Bvh(W.build_scene(spec))
+    ds = zo.DeviceScene(device, prod)
+    t = zo.Target(ds, spec.width, spec.height)
+    t.set_tuning(samples_per_job=2, split_percent=100)
+    for ci in (0.5, 16777214.0):
+        p = zo_params(spec, current_iteration=ci)
+        t.set_state(np.zeros((24 * 24, 4), np.float32), np.arange(24 * 24, dtype=np.uint32))
+        t.render(p, spec.spp)
+        acc, rng = np.zeros((24 * 24, 4), np.float32), O.seed_rng(24 * 24)
+        fb_o, _ = O.OracleScene(prod).render(O.ConstantParams.from_buffer_copy(bytes(p)),
+                                             spec.spp, acc, rng)
+        assert same_bits(t.accumulation(), acc), ci
+        assert same_bits(t.framebuffer(), fb_o), ci
+
+
+def test_job_handoff_survives_launch_tag_wrap(device):
+    """Pixels handed between jobs are tagged with a 16-bit launch counter
+    that wraps after 65,535 renders; 65,600 two-sample renders in two jobs
+    each equal one render of all their samples in one job per pixel."""
+    from helpers import zo_params
+    spec = W.cornell(width=8, height=8, spp=2)
+    prod = zo.Bvh(W.build_scene(spec))
+    ds = zo.DeviceScene(device, prod)
+    a = zo.Target(ds, 8, 8)
+    a.set_tuning(samples_per_job=1, split_percent=100)
+    renders = 65600
+    print(f"[wrap] {renders} renders", flush=True)
+    for k in range(renders):
+        a.render(zo_params(spec, current_iteration=float(2 * k)), 2, wait=(k % 4096 == 4095))
+    a.wait()
+    b = zo.Target(ds, 8, 8)
+    b.set_tuning(samples_per_job=0)
+    b.render(zo_params(spec), 2 * renders)
+    assert same_bits(a.accumulation(), b.accumulation())
+    assert np.array_equal(a.rng_state(), b.rng_state())
+    assert same_bits(a.framebuffer(), b.framebuffer())

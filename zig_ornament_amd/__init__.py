@@ -9,20 +9,27 @@ named after the reference's public API (src/ornament.zig):
                      render, getFrameBuffer, deinit} (wgpu_backend/path_tracer.zig)
     Device, DeviceScene, Target
                      the thin device ABI (orn_device_open, orn_scene_upload,
-                     orn_target_create, orn_render, orn_trace_rays)
+                     orn_target_create, orn_render, orn_trace_rays,
+                     orn_target_autotune)
+    Batch            many images of many scenes in one launch (orn_batch_*)
 
 Everything computes in the HIP library; nothing here falls back to Python.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
 from . import _native as N
 from ._native import OrnamentError, check, lib
 
-__all__ = ["Scene", "Bvh", "PathTracer", "Device", "DeviceScene", "Target", "OrnamentError",
-           "MaterialType", "NODE_DTYPE", "MATERIAL_DTYPE", "HIT_DTYPE", "scene_desc_arrays",
-           "lib"]
+__all__ = ["Scene", "Bvh", "PathTracer", "Device", "DeviceScene", "Target", "Batch",
+           "OrnamentError", "MaterialType", "NODE_DTYPE", "MATERIAL_DTYPE", "HIT_DTYPE",
+           "scene_desc_arrays", "lib", "TUNING_CACHE"]
+
+# Autotuner results measured on MI355X (gfx950) for the benchmark scene
+# classes; orn_target_autotune / orn_batch_autotune read and extend it.
+TUNING_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
 
 class MaterialType:  # material.zig:4-9
@@ -364,18 +371,22 @@ class DeviceScene:
             pass
 
 
-def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16):
+def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16,
+                   samples_per_job=16, split_percent=0):
     t = N.Tuning()
     t.regroup_lanes = int(regroup_lanes)
     t.workgroups_per_cu = int(workgroups_per_cu)
     t.bvh_width = int(bvh_width)
     t.leaf_regroup_lanes = int(leaf_regroup_lanes)
+    t.samples_per_job = int(samples_per_job)
+    t.split_percent = int(split_percent)
     return t
 
 
 def _tuning_dict(t):
     return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
-            "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes}
+            "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes,
+            "samples_per_job": t.samples_per_job, "split_percent": t.split_percent}
 
 
 class Target:
@@ -400,8 +411,10 @@ class Target:
     def wait(self):
         check(lib().orn_target_wait(self._h), "orn_target_wait")
 
-    def set_tuning(self, regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16):
-        t = _tuning_struct(regroup_lanes, workgroups_per_cu, bvh_width, leaf_regroup_lanes)
+    def set_tuning(self, regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16,
+                   samples_per_job=16, split_percent=0):
+        t = _tuning_struct(regroup_lanes, workgroups_per_cu, bvh_width, leaf_regroup_lanes,
+                           samples_per_job, split_percent)
         check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
 
     def tuning(self):

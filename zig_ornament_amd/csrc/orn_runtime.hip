@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <cmath>
 #include <cstdlib>
 #include <new>
 #include <string>
@@ -37,6 +38,7 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kTile = 16;  // kTile * kTile == kBlock
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kMaxStack = 63;  // the reference's private stack has 64 slots
+constexpr uint64_t kMaxPixels = 1ull << 28;  // per target or batch: 4 GiB of accumulation
 
 struct KernelParams {
     orn_constant_params p;  // single image; unused by batch launches
@@ -47,7 +49,11 @@ struct KernelParams {
     uint32_t regroup;       // waiting lanes that trigger a shading round
     uint32_t leaf_regroup;  // lanes at a leaf that trigger a leaf round
     uint32_t root;          // TLAS root reference (single image)
-    uint32_t pad;
+    uint32_t chunk;         // samples per job of a split pixel (single image)
+    uint32_t chunks;        // jobs per split pixel (single image)
+    uint32_t epoch;         // tag of this launch in the stage words (single image)
+    uint32_t split_pct;     // percentage of each band's pixels that are split
+    uint32_t pixels;        // pixels in the pixel buffers (at most 2^28)
 };
 
 // One image of a batch launch. All images of a batch share one device scene
@@ -56,35 +62,77 @@ struct BatchItem {
     orn_constant_params p;  // as in a single render() of this image
     uint32_t iterations;
     uint32_t tiles_x;
-    uint32_t px_base;  // first pixel of the image in the pooled buffers
-    uint32_t root;     // TLAS root reference of the image's scene
-    uint32_t tex_base; // first texture of the image's scene in the shared table
-    uint32_t pad[3];
+    uint32_t px_base;   // first pixel of the image in the pooled buffers
+    uint32_t root;      // TLAS root reference of the image's scene
+    uint32_t tex_base;  // first texture of the image's scene in the shared table
+    uint32_t chunk;     // samples per job of a split pixel
+    uint32_t chunks;    // jobs per split pixel
+    uint32_t epoch;     // tag of this launch in the stage words
 };
 static_assert(sizeof(BatchItem) == 176, "BatchItem layout");
 
 // Tile work list of a batch launch: band b owns entries
 // [band_begin[b], band_begin[b] + band_tiles[b]) of `tiles`, each an
-// (image, tile of the image) pair.
+// (image, tile of the image) pair; max_chunks is the largest jobs-per-pixel
+// count of the images.
 struct BatchTiles {
     uint32_t band_begin[kXcds];
     uint32_t band_tiles[kXcds];
+    uint32_t max_chunks;
+    uint32_t pad[3];
     uint2 tiles[1];  // band_begin[kXcds-1] + band_tiles[kXcds-1] entries
 };
 
 enum LaneMode : uint32_t { kTraverse = 0u, kWaiting = 1u, kDone = 2u };
 
+// Pixel state that moves between lanes (and XCDs) within one launch is read
+// and written with device-coherent accesses (they bypass the per-XCD L2s),
+// so no cache flush is needed to hand a pixel from one job to the next.
+__device__ __forceinline__ uint32_t load_coherent(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float load_coherent(const float *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_coherent(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// 16-byte accumulation entries go through a buffer descriptor, whose
+// cache-policy operand (sc1: device scope) makes one 16-byte access coherent.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kDeviceScope = 16;                    // cache policy sc1
+constexpr int kBufferWord3 = 0x00020000;            // raw 32-bit buffer, gfx9 family
+__device__ __forceinline__ float4 load_coherent(__amdgpu_buffer_rsrc_t r, uint32_t i)
+{
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, i * 16u, 0, kDeviceScope);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+__device__ __forceinline__ void store_coherent(__amdgpu_buffer_rsrc_t r, uint32_t i, float4 a)
+{
+    const u32x4 v = {__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(a.z), __float_as_uint(a.w)};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, i * 16u, 0, kDeviceScope);
+}
+
 // pathtracer.wgsl:64-79 + 95-161 + 81-93, every pixel over all iterations.
 //
-// Persistent lanes: the work is cut into 16x16 pixel tiles, the tiles into
-// one band per XCD (workgroup b runs on XCD b % 8), and each band is a work
-// queue of pixels. A lane renders one pixel's samples back to back (the
-// per-pixel RNG chain is serial), writes the pixel out and takes the next
-// pixel of its wave's band; a wave whose band is empty moves on to the next
-// band, so uneven bands even out.
+// Jobs: the work is cut into 16x16 pixel tiles, the tiles into one band per
+// XCD (workgroup b runs on XCD b % 8), and each band is a queue of jobs. The
+// first pixels of a band are one job each: a lane renders all of the pixel's
+// samples back to back (its RNG chain is serial, random.wgsl:3-9). The last
+// split_pct percent of the band's pixels are split into jobs of `chunk`
+// consecutive samples, queued all first jobs, then all second jobs, and so
+// on, so that when the queue runs dry every lane is at most one short job
+// from the end instead of a whole pixel. A split job starts from the
+// accumulation and RNG state its predecessor left in memory; its pixel's
+// stage word says how many jobs of this launch have finished (a lane that
+// draws a job before its predecessor has finished waits for it). A wave
+// whose band is empty moves on to the next band, so uneven bands even out.
 // Within a wave, a lane alternates between traversing its current ray one
 // node per loop iteration and a shading round (hit shading, accumulation,
-// next camera path, next pixel). A lane whose traversal ended waits; the wave
+// next camera path, next job). A lane whose traversal ended waits; the wave
 // runs a shading round only when at least `regroup` lanes wait (or none is
 // traversing), so node visits run with most lanes busy and the long shading
 // code runs for a batch of lanes at once. Each pixel's own sequence of
@@ -100,15 +148,21 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                                                         float4 *__restrict__ acc_buf,
                                                         uint32_t *__restrict__ rng_buf,
                                                         float4 *__restrict__ fb,
+                                                        uint32_t *__restrict__ stage,
                                                         uint32_t *__restrict__ queues)
 {
     extern __shared__ uint32_t lds_stack[];
     uint32_t *stack = lds_stack + threadIdx.x;
     const uint32_t lane = threadIdx.x % 64u;
+    const __amdgpu_buffer_rsrc_t acc_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(acc_buf, 0, static_cast<int>(kp.pixels * 16u), kBufferWord3);
 
     uint32_t x = 0, y = 0;
     uint32_t pix = 0;   // pixel index in the pixel buffers
     uint32_t item = 0;  // image of the lane's pixel (BATCH)
+    uint32_t job = 0;   // index of the lane's job within its pixel
+    uint32_t lane_chunks = 1;  // jobs of the lane's pixel (1: a whole pixel)
+    uint32_t release_pix = 0, release_tag = 0; // stage word still to be written
     uint32_t rng = 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float cur = 0.0f;
@@ -119,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     Traversal tr;
     uint32_t mode = kWaiting;
     bool need_pixel = true;
+    bool blocked = false;  // holds a job whose predecessor has not finished
     // per-image values a lane keeps while it works on a pixel (BATCH); a
     // single image takes them from the kernel arguments
     float lane_t_min = 0.0f;
@@ -150,11 +205,29 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         thr = mk(1.0f, 1.0f, 1.0f);
         bounce = 0;
     };
-    // pathtracer.wgsl:81-93 and the state write-back of one finished pixel
-    auto finish_pixel = [&]() {
-        const orn_constant_params &p = params();
-        acc_buf[pix] = acc;
+    // stage word a finished job leaves for its successor
+    auto stage_tag = [&](uint32_t jobs_done) {
+        return ((BATCH ? items[item].epoch : kp.epoch) << 16) | jobs_done;
+    };
+    auto job_ready = [&]() { return job == 0u || load_coherent(stage + pix) == stage_tag(job); };
+    // End of a job: the state goes back to memory; the last job of a pixel
+    // also writes the framebuffer (pathtracer.wgsl:81-93), the others
+    // release the pixel to its next job.
+    auto end_job = [&]() {
+        if (job + 1u < lane_chunks) {  // handed on; released at the wave's next flush
+            store_coherent(acc_rsrc, pix, acc);
+            store_coherent(rng_buf + pix, rng);
+            if (release_tag != 0u) {  // a release is still pending: write it now
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                store_coherent(stage + release_pix, release_tag);
+            }
+            release_pix = pix;
+            release_tag = stage_tag(job + 1u);
+            return;
+        }
+        acc_buf[pix] = acc;  // the pixel's last job: read next by a later launch
         rng_buf[pix] = rng;
+        const orn_constant_params &p = params();
         float4 o;
         o.x = fminf(fmaxf(acc.x / cur, 0.0f), 1.0f);
         o.y = fminf(fmaxf(acc.y / cur, 0.0f), 1.0f);
@@ -168,8 +241,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         fb[base + row * p.width + x] = o;
     };
     // Accumulate finished samples, finish samples that need no traversal
-    // (depth exhausted), and set up the next traversal; a finished pixel
-    // asks for a new one.
+    // (depth exhausted), and set up the next traversal; a finished job asks
+    // for a new one.
     auto next_traversal = [&](bool done, f3 color) {
         for (;;) {
             if (done) {
@@ -183,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                     acc = make_float4(color.x, color.y, color.z, 1.0f);
                 }
                 if (--left == 0u) {
-                    finish_pixel();
+                    end_job();
                     need_pixel = true;
                     mode = kWaiting;
                     return;
@@ -200,7 +273,43 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
             return;
         }
     };
-    // pixels per band
+    // Start the lane's job from the state in memory. current_iteration + the
+    // job's first sample is exact (integer-valued, below 2^24: the host
+    // chunks a render only then), so `cur` equals the value the reference's
+    // per-sample increments reach.
+    auto begin_job = [&]() {
+        const orn_constant_params &p = params();
+        if (job == 0u) {  // state left by an earlier launch
+            rng = rng_buf[pix];
+            acc = acc_buf[pix];
+        } else {  // state handed on within this launch
+            rng = load_coherent(rng_buf + pix);
+            acc = load_coherent(acc_rsrc, pix);
+        }
+        const uint32_t iters = BATCH ? items[item].iterations : kp.iterations;
+        const uint32_t spp = iters > 1u ? iters : 1u;
+        const uint32_t chunk = lane_chunks == 1u ? spp : (BATCH ? items[item].chunk : kp.chunk);
+        const uint32_t first = job * chunk;
+        cur = first == 0u ? p.current_iteration : p.current_iteration + static_cast<float>(first);
+        left = min(chunk, spp - first);
+        start_sample();
+        next_traversal(false, thr);
+    };
+    // Stage words of finished jobs are written once the job's state stores
+    // have completed: a wait on all of the wave's memory operations, which by
+    // the next shading round have long been done, so the wait is short.
+    auto flush_releases = [&]() {
+        if (__ballot(release_tag != 0u) != 0u) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (release_tag != 0u) {
+                store_coherent(stage + release_pix, release_tag);
+                release_tag = 0u;
+            }
+        }
+    };
+    // jobs per band: band_px - split whole pixels, then split x max_chunks
+    const uint32_t max_chunks = BATCH ? work->max_chunks : kp.chunks;
+    const uint32_t split_pct = kp.split_pct;
     auto band_pixels = [&](uint32_t b) -> uint32_t {
         if constexpr (BATCH) return work->band_tiles[b] * (kTile * kTile);
         const uint32_t first = b * kp.tiles_per_xcd;
@@ -208,8 +317,10 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     };
 
     uint32_t band = blockIdx.x % kXcds;  // wave-uniform
-    uint32_t bands_left = kXcds - 1u;  // bands to visit after this one
+    uint32_t bands_left = kXcds - 1u;    // bands to visit after this one
     uint32_t band_px = band_pixels(band);
+    uint32_t band_split = band_px / 100u * split_pct + band_px % 100u * split_pct / 100u;
+    uint32_t band_jobs = band_px - band_split + band_split * max_chunks;
     const uint32_t regroup = kp.regroup;
     const uint32_t leaf_regroup = kp.leaf_regroup;
     for (;;) {
@@ -230,30 +341,39 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         if (waiting == 0u && traversing == 0u) break;
         if (static_cast<uint32_t>(__popcll(waiting)) < regroup && traversing != 0u) continue;
         // shading round
+        flush_releases();
         if (mode == kWaiting && !need_pixel) {
-            bounce++;
-            if (!tr.hit) {
-                const f3 ud = normalize(ray.d);
-                const float tt = 0.5f * (ud.y + 1.0f);
-                const float a = (1.0f - tt) * 1.0f;
-                next_traversal(true, thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f));
+            if (blocked) {
+                if (job_ready()) {
+                    blocked = false;
+                    begin_job();
+                }
             } else {
-                Ray next;
-                f3 w;
-                const uint32_t tex_count = params().textures_count;
-                const uint32_t tex_base = BATCH ? items[item].tex_base : 0u;
-                if (shade<F>(sc, tex_count, tex_base, ray, tr.h, rng, next, w)) {
-                    ray = next;
-                    thr = thr * w;
-                    next_traversal(false, thr);
+                bounce++;
+                if (!tr.hit) {
+                    const f3 ud = normalize(ray.d);
+                    const float tt = 0.5f * (ud.y + 1.0f);
+                    const float a = (1.0f - tt) * 1.0f;
+                    next_traversal(true, thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f));
                 } else {
-                    next_traversal(true, thr * w);
+                    Ray next;
+                    f3 w;
+                    const uint32_t tex_count = params().textures_count;
+                    const uint32_t tex_base = BATCH ? items[item].tex_base : 0u;
+                    if (shade<F>(sc, tex_count, tex_base, ray, tr.h, rng, next, w)) {
+                        ray = next;
+                        thr = thr * w;
+                        next_traversal(false, thr);
+                    } else {
+                        next_traversal(true, thr * w);
+                    }
                 }
             }
         }
-        // lanes without a pixel take the next ones of the wave's band, one
-        // atomic per wave; pixels of partial edge tiles are skipped; a band
-        // that runs out sends the wave on to the next one
+        // lanes without a job take the next ones of the wave's band, one
+        // atomic per wave; pixels of partial edge tiles (and, in a batch,
+        // jobs past an image's last) are skipped; a band that runs out
+        // sends the wave on to the next one
         for (;;) {
             const uint64_t needy = __ballot(mode == kWaiting && need_pixel);
             if (needy == 0u) break;
@@ -264,54 +384,66 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
             base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), leader));
             if (mode == kWaiting && need_pixel) {
                 const uint32_t q = base + static_cast<uint32_t>(__popcll(needy & ((1ull << lane) - 1ull)));
-                if (q >= band_px) {
+                if (q >= band_jobs) {
                     if (bands_left == 0u) mode = kDone;  // every band has run out
                 } else {
-                    const uint32_t within = q % (kTile * kTile);
-                    uint32_t tile, tiles_x, w, h;
+                    // whole pixel, or job c of split pixel qp
+                    const uint32_t bulk = band_px - band_split;
+                    uint32_t c = 0, qp = q;
+                    if (q >= bulk) {
+                        c = (q - bulk) / band_split;
+                        qp = bulk + (q - bulk - c * band_split);
+                    }
+                    const uint32_t within = qp % (kTile * kTile);
+                    uint32_t tile, tiles_x, w, h, chunks;
                     if constexpr (BATCH) {
-                        const uint2 e = work->tiles[work->band_begin[band] + q / (kTile * kTile)];
+                        const uint2 e = work->tiles[work->band_begin[band] + qp / (kTile * kTile)];
                         item = e.x;
                         tile = e.y;
                         tiles_x = items[item].tiles_x;
                         w = items[item].p.width;
                         h = items[item].p.height;
+                        chunks = items[item].chunks;
                     } else {
-                        tile = band * kp.tiles_per_xcd + q / (kTile * kTile);
+                        tile = band * kp.tiles_per_xcd + qp / (kTile * kTile);
                         tiles_x = kp.tiles_x;
                         w = kp.p.width;
                         h = kp.p.height;
+                        chunks = kp.chunks;
                     }
                     x = (tile % tiles_x) * kTile + within % kTile;
                     y = (tile / tiles_x) * kTile + within / kTile;
-                    if (x < w && y < h) {
-                        const orn_constant_params &p = params();
+                    const bool split = qp >= bulk;
+                    if (x < w && y < h && (!split || c < chunks)) {
                         pix = y * w + x;
-                        uint32_t spp = kp.iterations;
+                        lane_chunks = split ? chunks : 1u;
                         if constexpr (BATCH) {
+                            const orn_constant_params &p = params();
                             pix += items[item].px_base;
-                            spp = items[item].iterations;
                             lane_t_min = p.ray_cast_epsilon;
                             lane_depth = p.depth;
                             lane_root = items[item].root;
                         }
-                        rng = rng_buf[pix];
-                        acc = acc_buf[pix];
-                        cur = p.current_iteration;
-                        left = spp > 1u ? spp : 1u;
+                        job = c;
                         need_pixel = false;
-                        start_sample();
-                        next_traversal(false, thr);
+                        if (job_ready()) {
+                            begin_job();
+                        } else {
+                            blocked = true;
+                        }
                     }
                 }
             }
-            if (base + count > band_px && bands_left != 0u) {  // the band ran out (wave-uniform)
+            if (base + count > band_jobs && bands_left != 0u) {  // the band ran out (wave-uniform)
                 band = band + 1u == kXcds ? 0u : band + 1u;
                 bands_left--;
                 band_px = band_pixels(band);
+                band_split = band_px / 100u * split_pct + band_px % 100u * split_pct / 100u;
+                band_jobs = band_px - band_split + band_split * max_chunks;
             }
         }
     }
+    flush_releases();  // no successor may wait on a release this wave holds
 }
 
 __global__ void reset_queues_kernel(uint32_t *q)
@@ -367,6 +499,12 @@ __global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+__global__ void fill_u32_kernel(uint32_t *a, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void seed_rng_kernel(uint32_t *rng, uint32_t n, uint32_t base)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -419,7 +557,9 @@ struct orn_target {
     float4 *fb;
     float4 *acc;
     uint32_t *rng;
-    uint32_t *queues;  // one pixel-queue counter per XCD band
+    uint32_t *stage;   // per pixel: launch tag << 16 | jobs finished in that launch
+    uint32_t epoch;    // tag of the latest launch
+    uint32_t *queues;  // one job-queue counter per XCD band
     uint32_t grid;     // persistent workgroups of the render kernel
     hipEvent_t ev_start, ev_stop;
     bool timed;
@@ -667,7 +807,38 @@ orn_tuning default_tuning()
     t.workgroups_per_cu = 0;  // as many as can be resident at once
     t.bvh_width = 2;
     t.leaf_regroup_lanes = 16;
+    t.samples_per_job = 16;
+    t.split_percent = 0;
     return t;
+}
+
+bool tuning_valid(const orn_tuning *t)
+{
+    return t && t->regroup_lanes >= 1 && t->regroup_lanes <= 64 && t->workgroups_per_cu <= 64 &&
+           (t->bvh_width == 2u || t->bvh_width == 4u) && t->leaf_regroup_lanes >= 1 &&
+           t->leaf_regroup_lanes <= 64 && t->samples_per_job <= 0xffffu && t->split_percent <= 100u;
+}
+
+// Jobs of one pixel's render: `chunk` samples each, `chunks` of them. A render
+// is cut only when current_iteration + k stays exact in f32 for every k up
+// to its sample count (a whole number, below 2^24 at the end), and the job
+// counter of a band fits 31 bits.
+void plan_jobs(uint32_t iterations, float current_iteration, uint32_t samples_per_job, uint64_t band_pixels,
+               uint32_t *chunk, uint32_t *chunks)
+{
+    const uint32_t spp = iterations > 1u ? iterations : 1u;
+    const bool exact = current_iteration >= 0.0f && std::floor(current_iteration) == current_iteration &&
+                       double(current_iteration) + spp <= 16777216.0;
+    uint32_t c = (samples_per_job == 0u || !exact) ? spp : std::min(samples_per_job, spp);
+    while (true) {
+        const uint32_t n = (spp + c - 1u) / c;
+        if (n <= 0xffffu && uint64_t(n) * std::max<uint64_t>(band_pixels, 1) < (1ull << 31)) {
+            *chunk = c;
+            *chunks = n;
+            return;
+        }
+        c = c >= spp / 2u ? spp : c * 2u;
+    }
 }
 
 // Buffers and sizes of one render launch.
@@ -678,6 +849,7 @@ struct LaunchArgs {
     float4 *acc;
     uint32_t *rng;
     float4 *fb;
+    uint32_t *stage;
     uint32_t *queues;
     uint32_t total_tiles;
 };
@@ -700,7 +872,8 @@ hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_
     uint32_t grid = cus * per_cu;
     grid = std::min(grid, blocks_for(a.total_tiles, kXcds) * kXcds);
     grid = std::max(kXcds, grid / kXcds * kXcds);
-    render_kernel<F, W, BATCH><<<grid, kBlock, lds, st>>>(ds, a.kp, a.items, a.work, a.acc, a.rng, a.fb, a.queues);
+    render_kernel<F, W, BATCH>
+        <<<grid, kBlock, lds, st>>>(ds, a.kp, a.items, a.work, a.acc, a.rng, a.fb, a.stage, a.queues);
     return hipGetLastError();
 }
 
@@ -998,7 +1171,7 @@ orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t heigh
     if (!s) return ORN_ERROR_INVALID_HANDLE;
     if (!out || width == 0 || height == 0) return ORN_ERROR_INVALID_VALUE;
     const uint64_t n = uint64_t(width) * height;
-    if (n > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
+    if (n > kMaxPixels) return ORN_ERROR_INVALID_VALUE;
     *out = nullptr;
     ORN_TRY(hipSetDevice(s->ctx->device));
     orn_target *t = new (std::nothrow) orn_target();
@@ -1017,12 +1190,14 @@ orn_status orn_target_create(orn_device_scene *s, uint32_t width, uint32_t heigh
     if ((e = hipMalloc(&t->acc, n * sizeof(float4))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&t->rng, n * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&t->queues, 64 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&t->stage, n * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipEventCreate(&t->ev_start)) != hipSuccess) return fail(e);
     if ((e = hipEventCreate(&t->ev_stop)) != hipSuccess) return fail(e);
     hipStream_t st = s->ctx->stream;
     const uint32_t nn = static_cast<uint32_t>(n);
     zero_kernel<<<blocks_for(nn, 256), 256, 0, st>>>(t->fb, t->acc, nn);
     seed_rng_kernel<<<blocks_for(nn, 256), 256, 0, st>>>(t->rng, nn, seed * nn);
+    fill_u32_kernel<<<blocks_for(nn, 256), 256, 0, st>>>(t->stage, nn, 0u);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
     *out = t;
@@ -1038,6 +1213,7 @@ orn_status orn_target_destroy(orn_target *t)
     if (t->acc) (void)hipFree(t->acc);
     if (t->rng) (void)hipFree(t->rng);
     if (t->queues) (void)hipFree(t->queues);
+    if (t->stage) (void)hipFree(t->stage);
     if (t->ev_start) (void)hipEventDestroy(t->ev_start);
     if (t->ev_stop) (void)hipEventDestroy(t->ev_stop);
     orn_device_scene *s = t->scene;
@@ -1053,7 +1229,7 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
     orn_device_scene *s = t->scene;
     ORN_TRY(hipSetDevice(s->ctx->device));
-    KernelParams kp;
+    KernelParams kp{};
     kp.p = *params;
     kp.iterations = iterations;
     kp.tiles_x = blocks_for(t->width, kTile);
@@ -1063,12 +1239,21 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     kp.leaf_regroup = t->tuning.leaf_regroup_lanes;
     const uint32_t width = t->tuning.bvh_width == 2u ? 2u : 4u;
     kp.root = width == 4u ? s->ds.tlas4_root : s->ds.tlas2_root;
-    kp.pad = 0;
+    plan_jobs(iterations, params->current_iteration, t->tuning.samples_per_job,
+              uint64_t(kp.tiles_per_xcd) * kTile * kTile, &kp.chunk, &kp.chunks);
     DevScene ds = s->ds;
     ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
     const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
     hipStream_t st = s->ctx->stream;
-    LaunchArgs a{kp, nullptr, nullptr, t->acc, t->rng, t->fb, t->queues, kp.tiles};
+    const uint32_t n = t->width * t->height;
+    if (++t->epoch > 0xffffu) {  // tags wrap: clear the stage words
+        fill_u32_kernel<<<blocks_for(n, 256), 256, 0, st>>>(t->stage, n, 0u);
+        t->epoch = 1;
+    }
+    kp.epoch = t->epoch;
+    kp.split_pct = t->tuning.split_percent;
+    kp.pixels = n;
+    LaunchArgs a{kp, nullptr, nullptr, t->acc, t->rng, t->fb, t->stage, t->queues, kp.tiles};
     ORN_TRY(hipEventRecord(t->ev_start, st));
     reset_queues_kernel<<<1, 64, 0, st>>>(t->queues);
     ORN_TRY(launch_render<false>(s->features, width, t->tuning.workgroups_per_cu,
@@ -1096,10 +1281,7 @@ orn_status orn_render(orn_target *t, const orn_constant_params *params, uint32_t
 orn_status orn_target_set_tuning(orn_target *t, const orn_tuning *tuning)
 {
     if (!t) return ORN_ERROR_INVALID_HANDLE;
-    if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64 ||
-        (tuning->bvh_width != 2u && tuning->bvh_width != 4u) || tuning->leaf_regroup_lanes < 1 ||
-        tuning->leaf_regroup_lanes > 64)
-        return ORN_ERROR_INVALID_VALUE;
+    if (!tuning_valid(tuning)) return ORN_ERROR_INVALID_VALUE;
     t->tuning = *tuning;
     return ORN_OK;
 }
@@ -1378,6 +1560,8 @@ struct orn_batch {
     std::vector<uint64_t> work_cost;  // per-image costs the current work list was ordered by
     float4 *fb, *acc;
     uint32_t *rng;
+    uint32_t *stage;  // per pixel: launch tag << 16 | jobs finished in that launch
+    uint32_t epoch;   // tag of the latest launch
     uint32_t *queues;
     orn_tuning tuning;
     hipGraph_t graph;
@@ -1404,6 +1588,7 @@ void free_batch(orn_batch *b)
     if (b->acc) (void)hipFree(b->acc);
     if (b->rng) (void)hipFree(b->rng);
     if (b->queues) (void)hipFree(b->queues);
+    if (b->stage) (void)hipFree(b->stage);
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
     if (b->ev_stop) (void)hipEventDestroy(b->ev_stop);
     delete b;
@@ -1413,10 +1598,11 @@ void free_batch(orn_batch *b)
 // Orders the tile work list by the images' expected cost per pixel
 // (samples x (depth + 1)), most expensive first, and deals it out to the
 // eight bands in runs of 4 tiles, so every band starts on the long pixels
-// and all bands hold about the same work.
+// and all bands hold about the same work. The last entry of `cost` is the
+// largest jobs-per-pixel count of the images.
 void build_work(orn_batch *b, const std::vector<uint64_t> &cost)
 {
-    std::vector<uint32_t> order(b->n);
+    std::vector<uint32_t> order(b->n);  // images, costliest first
     for (size_t i = 0; i < b->n; i++) order[i] = static_cast<uint32_t>(i);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return cost[a] > cost[c]; });
     std::vector<std::vector<uint2>> band(kXcds);
@@ -1424,6 +1610,7 @@ void build_work(orn_batch *b, const std::vector<uint64_t> &cost)
     uint32_t dealt = 0;
     for (uint32_t i : order)
         for (uint32_t t = 0; t < b->tiles[i]; t++, dealt++) band[(dealt / kRun) % kXcds].push_back(make_uint2(i, t));
+    b->h_work->max_chunks = static_cast<uint32_t>(cost[b->n]);
     uint32_t at = 0;
     for (uint32_t k = 0; k < kXcds; k++) {
         b->h_work->band_begin[k] = at;
@@ -1451,7 +1638,9 @@ orn_status capture_batch(orn_batch *b)
     KernelParams kp{};
     kp.regroup = b->tuning.regroup_lanes;
     kp.leaf_regroup = b->tuning.leaf_regroup_lanes;
-    LaunchArgs a{kp, b->d_items, b->d_work, b->acc, b->rng, b->fb, b->queues, b->total_tiles};
+    kp.split_pct = b->tuning.split_percent;
+    kp.pixels = b->total_px;
+    LaunchArgs a{kp, b->d_items, b->d_work, b->acc, b->rng, b->fb, b->stage, b->queues, b->total_tiles};
     ORN_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     hipError_t e = hipMemcpyAsync(b->d_items, b->h_items, b->n * sizeof(BatchItem), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
@@ -1500,7 +1689,7 @@ orn_status orn_batch_create(orn_ctx *ctx, const orn_batch_item *items, size_t n,
         px += uint64_t(items[i].width) * items[i].height;
         tiles += uint64_t(blocks_for(items[i].width, kTile)) * blocks_for(items[i].height, kTile);
     }
-    if (px > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
+    if (px > kMaxPixels) return ORN_ERROR_INVALID_VALUE;
     MergedScene m;
     uint32_t slots = 0;
     orn_status st = merge_scenes(items, n, m, &slots);
@@ -1548,11 +1737,13 @@ orn_status orn_batch_create(orn_ctx *ctx, const orn_batch_item *items, size_t n,
     if ((e = hipMalloc(&b->acc, size_t(base) * sizeof(float4))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&b->rng, size_t(base) * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&b->queues, 64 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&b->stage, size_t(base) * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipEventCreate(&b->ev_start)) != hipSuccess) return fail(e);
     if ((e = hipEventCreate(&b->ev_stop)) != hipSuccess) return fail(e);
     std::memset(b->h_items, 0, n * sizeof(BatchItem));
     hipStream_t stream = ctx->stream;
     zero_kernel<<<blocks_for(base, 256), 256, 0, stream>>>(b->fb, b->acc, base);
+    fill_u32_kernel<<<blocks_for(base, 256), 256, 0, stream>>>(b->stage, base, 0u);
     for (size_t i = 0; i < n; i++) {
         const uint32_t nn = items[i].width * items[i].height;
         seed_rng_kernel<<<blocks_for(nn, 256), 256, 0, stream>>>(b->rng + b->px_base[i], nn, items[i].seed * nn);
@@ -1582,7 +1773,12 @@ orn_status orn_batch_render_async(orn_batch *b, const orn_constant_params *param
     // the previous launch may still read the pinned staging buffers
     ORN_TRY(hipStreamSynchronize(st));
     const bool w4 = b->tuning.bvh_width == 4u;
-    std::vector<uint64_t> cost(b->n);
+    if (++b->epoch > 0xffffu) {  // tags wrap: clear the stage words
+        fill_u32_kernel<<<blocks_for(b->total_px, 256), 256, 0, st>>>(b->stage, b->total_px, 0u);
+        b->epoch = 1;
+    }
+    std::vector<uint64_t> cost(b->n + 1);
+    uint32_t max_chunks = 1;
     for (size_t i = 0; i < b->n; i++) {
         BatchItem &it = b->h_items[i];
         it.p = params[i];
@@ -1591,8 +1787,13 @@ orn_status orn_batch_render_async(orn_batch *b, const orn_constant_params *param
         it.px_base = b->px_base[i];
         it.root = w4 ? b->root4[i] : b->root2[i];
         it.tex_base = b->tex_base[i];
+        plan_jobs(iterations[i], params[i].current_iteration, b->tuning.samples_per_job,
+                  uint64_t(b->total_tiles) * kTile * kTile, &it.chunk, &it.chunks);
+        it.epoch = b->epoch;
+        max_chunks = std::max(max_chunks, it.chunks);
         cost[i] = uint64_t(std::max(iterations[i], 1u)) * (uint64_t(params[i].depth) + 1u);
     }
+    cost[b->n] = max_chunks;
     if (cost != b->work_cost) {
         build_work(b, cost);
         ORN_TRY(hipMemcpyAsync(b->d_work, b->h_work, b->work_bytes, hipMemcpyHostToDevice, st));
@@ -1626,10 +1827,7 @@ orn_status orn_batch_render(orn_batch *b, const orn_constant_params *params, con
 orn_status orn_batch_set_tuning(orn_batch *b, const orn_tuning *tuning)
 {
     if (!b) return ORN_ERROR_INVALID_HANDLE;
-    if (!tuning || tuning->regroup_lanes < 1 || tuning->regroup_lanes > 64 || tuning->workgroups_per_cu > 64 ||
-        (tuning->bvh_width != 2u && tuning->bvh_width != 4u) || tuning->leaf_regroup_lanes < 1 ||
-        tuning->leaf_regroup_lanes > 64)
-        return ORN_ERROR_INVALID_VALUE;
+    if (!tuning_valid(tuning)) return ORN_ERROR_INVALID_VALUE;
     if (std::memcmp(tuning, &b->tuning, sizeof(orn_tuning)) != 0) {
         ORN_TRY(hipSetDevice(b->scene->ctx->device));
         ORN_TRY(hipStreamSynchronize(b->scene->ctx->stream));  // the graph may be running
@@ -1831,12 +2029,12 @@ bool cached_tuning(const char *path, const std::string &key, orn_tuning *t, floa
             else if (f.first == "leaf_regroup_lanes") r.leaf_regroup_lanes = v, ok[1] = true;
             else if (f.first == "workgroups_per_cu") r.workgroups_per_cu = v, ok[2] = true;
             else if (f.first == "bvh_width") r.bvh_width = v, ok[3] = true;
+            else if (f.first == "samples_per_job") r.samples_per_job = v;
+            else if (f.first == "split_percent") r.split_percent = v;
             else if (f.first == "ms") *ms = static_cast<float>(f.second);
         }
         if (!(ok[0] && ok[1] && ok[2] && ok[3])) return false;
-        if (r.regroup_lanes < 1 || r.regroup_lanes > 64 || r.leaf_regroup_lanes < 1 || r.leaf_regroup_lanes > 64 ||
-            r.workgroups_per_cu > 64 || (r.bvh_width != 2u && r.bvh_width != 4u))
-            return false;
+        if (!tuning_valid(&r)) return false;
         *t = r;
         return true;
     }
@@ -1852,6 +2050,8 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
         {"leaf_regroup_lanes", t.leaf_regroup_lanes},
         {"workgroups_per_cu", t.workgroups_per_cu},
         {"bvh_width", t.bvh_width},
+        {"samples_per_job", t.samples_per_job},
+        {"split_percent", t.split_percent},
         {"ms", ms}};
     bool found = false;
     for (auto &e : c)
@@ -1892,6 +2092,8 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
     static const uint32_t leaf[] = {1, 4, 8, 16, 32};
     static const uint32_t per_cu[] = {0, 1, 2, 3};
     static const uint32_t width[] = {2, 4};
+    static const uint32_t split[] = {25, 50, 100};
+    static const uint32_t job[] = {4, 8, 16, 32};
     auto sweep = [&](uint32_t orn_tuning::*field, const uint32_t *vals, size_t n) -> orn_status {
         for (size_t i = 0; i < n; i++) {
             orn_tuning t = cur;
@@ -1910,6 +2112,19 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
     if ((st = sweep(&orn_tuning::leaf_regroup_lanes, leaf, 5)) != ORN_OK) return st;
     if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
     if ((st = sweep(&orn_tuning::workgroups_per_cu, per_cu, 4)) != ORN_OK) return st;
+    // job splitting: both fields at once (they only pay off together)
+    for (uint32_t sp : split)
+        for (uint32_t jb : job) {
+            orn_tuning t = cur;
+            t.split_percent = sp;
+            t.samples_per_job = jb;
+            float ms = 0;
+            if ((st = measure(t, &ms)) != ORN_OK) return st;
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
     if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
     *best = cur;
     *best_ms = cur_ms;
