@@ -152,6 +152,54 @@ orn_status ornament_state_get_current_iteration(ornament_path_tracer *pt, float 
 orn_status ornament_path_tracer_constant_params(ornament_path_tracer *pt,
                                                 orn_constant_params *out);
 
+/* ------------------------------------------------------------------ */
+/* Scene files and images. The reference builds its scenes in code     */
+/* (glfw_example/examples.zig) and shows images in a window; these     */
+/* files make a scene and its render settings reproducible between      */
+/* machines and keep rendered images (Target.getFrameBuffer,            */
+/* wgpu_backend/buffers.zig:67-91) as float data.                       */
+/* ------------------------------------------------------------------ */
+
+/* Render settings stored beside a scene (State, state.zig:3-23). */
+typedef struct ornament_render_settings {
+    uint32_t width;
+    uint32_t height;
+    uint32_t iterations; /* samples per pixel of one render() */
+    uint32_t depth;
+    float gamma;
+    float ray_cast_epsilon;
+    uint32_t flip_y;
+    uint32_t seed; /* RNG seed of the target (orn_target_create) */
+} ornament_render_settings;
+
+/* Scene file (little-endian, "ORNSCENE" version 1): the camera, optional
+ * render settings, then every texture, material, sphere, mesh and mesh
+ * instance in creation order with its attach flag; meshes are stored as
+ * created (vertices, normals and uvs with their index arrays, model
+ * transform). Loading replays the scene factories, so a loaded scene builds
+ * the same BVH as the saved one. `settings` may be NULL (not stored / not
+ * wanted); `has_settings` reports whether the file held them. */
+orn_status ornament_scene_save(const ornament_scene *s, const ornament_render_settings *settings,
+                               const char *path);
+orn_status ornament_scene_load(const char *path, ornament_scene **out,
+                               ornament_render_settings *settings, int *has_settings);
+
+/* Images: `rgba` holds width * height float4 pixels, row 0 first; row 0 is
+ * the bottom of the picture (v = 0 in camera_get_ray, camera.wgsl:12-23),
+ * as a framebuffer rendered with flip_y = false holds it.
+ * PFM ("PF", RGB, little-endian, rows bottom to top; alpha is dropped). */
+orn_status ornament_image_write_pfm(const char *path, const float *rgba, uint32_t width,
+                                    uint32_t height);
+/* Reads a PFM ("PF" RGB or "Pf" grey, either byte order) into rgba (alpha
+ * 1); width and height are returned; `pixels` is the capacity of rgba (call
+ * with rgba = NULL to query the size). */
+orn_status ornament_image_read_pfm(const char *path, float *rgba, size_t pixels, uint32_t *width,
+                                   uint32_t *height);
+/* OpenEXR 2.0 scanline image, uncompressed, channels A, B, G, R as 32-bit
+ * float, rows top to bottom. */
+orn_status ornament_image_write_exr(const char *path, const float *rgba, uint32_t width,
+                                    uint32_t height);
+
 #ifdef __cplusplus
 }
 #endif

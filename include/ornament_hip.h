@@ -43,6 +43,8 @@ typedef int orn_status;
 #define ORN_ERROR_UNSUPPORTED_TEXTURE 9005 /* wgpu buffers.zig:316-336 asserts */
 #define ORN_ERROR_INVALID_HANDLE 9006      /* null or destroyed handle */
 #define ORN_ERROR_SIZE_MISMATCH 9007       /* destination smaller than the target */
+#define ORN_ERROR_IO 9008                  /* a file could not be opened, read or written */
+#define ORN_ERROR_INVALID_FILE 9009        /* a scene or image file is malformed */
 
 /* Human-readable name of a status ("InvalidValue", "EmptyScene", ...). */
 const char *orn_status_string(orn_status status);

@@ -25,7 +25,7 @@ from ._native import OrnamentError, check, lib
 
 __all__ = ["Scene", "Bvh", "PathTracer", "Device", "DeviceScene", "Target", "Batch",
            "OrnamentError", "MaterialType", "NODE_DTYPE", "MATERIAL_DTYPE", "HIT_DTYPE",
-           "scene_desc_arrays", "lib", "TUNING_CACHE"]
+           "scene_desc_arrays", "lib", "TUNING_CACHE", "write_pfm", "read_pfm", "write_exr"]
 
 # Autotuner results measured on MI355X (gfx950) for the benchmark scene
 # classes; orn_target_autotune / orn_batch_autotune read and extend it.
@@ -70,12 +70,34 @@ def _ptr(a):
 class Scene:
     """ornament.Scene (scene.zig:13-380). Objects are handles owned by the scene."""
 
-    def __init__(self):
-        h = C.c_void_p()
-        check(lib().ornament_scene_init(C.byref(h)), "Scene.init")
+    def __init__(self, _handle=None):
+        if _handle is None:
+            h = C.c_void_p()
+            check(lib().ornament_scene_init(C.byref(h)), "Scene.init")
+        else:
+            h = _handle
         self._h = h
         self._owned = True
         self._keep = []
+
+    # scene files -----------------------------------------------------------
+    def save(self, path, settings=None):
+        """ornament_scene_save: the scene (and optional render settings, a
+        dict of RenderSettings fields) to an ORNSCENE file."""
+        rs = None
+        if settings is not None:
+            rs = N.RenderSettings(**settings)
+        check(lib().ornament_scene_save(self._h, C.byref(rs) if rs is not None else None,
+                                        os.fsencode(path)), "ornament_scene_save")
+
+    @classmethod
+    def load(cls, path):
+        """ornament_scene_load: (Scene, settings dict or None)."""
+        h, rs, has = C.c_void_p(), N.RenderSettings(), C.c_int()
+        check(lib().ornament_scene_load(os.fsencode(path), C.byref(h), C.byref(rs), C.byref(has)),
+              "ornament_scene_load")
+        settings = {f: getattr(rs, f) for f, _ in N.RenderSettings._fields_} if has.value else None
+        return cls(_handle=h), settings
 
     @property
     def handle(self):
@@ -278,6 +300,35 @@ def scene_desc_arrays(arrays):
     d.textures = C.addressof(tex) if arrays.textures else None
     d.textures_len = len(arrays.textures)
     return d, keep
+
+
+def write_pfm(path, rgba, width, height):
+    """ornament_image_write_pfm: float RGBA pixels (row 0 = bottom) as PFM."""
+    a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+    if len(a) != width * height:
+        raise ValueError("rgba must hold width * height pixels")
+    check(lib().ornament_image_write_pfm(os.fsencode(path), _ptr(a), int(width), int(height)),
+          "ornament_image_write_pfm")
+
+
+def read_pfm(path):
+    """ornament_image_read_pfm: (rgba (h*w, 4) float32 with row 0 = bottom, width, height)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    p = os.fsencode(path)
+    check(lib().ornament_image_read_pfm(p, None, 0, C.byref(w), C.byref(h)), "ornament_image_read_pfm")
+    out = np.empty((w.value * h.value, 4), np.float32)
+    check(lib().ornament_image_read_pfm(p, _ptr(out), len(out), C.byref(w), C.byref(h)),
+          "ornament_image_read_pfm")
+    return out, w.value, h.value
+
+
+def write_exr(path, rgba, width, height):
+    """ornament_image_write_exr: float RGBA pixels (row 0 = bottom) as OpenEXR."""
+    a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+    if len(a) != width * height:
+        raise ValueError("rgba must hold width * height pixels")
+    check(lib().ornament_image_write_exr(os.fsencode(path), _ptr(a), int(width), int(height)),
+          "ornament_image_write_exr")
 
 
 class Device:

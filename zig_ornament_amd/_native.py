@@ -65,6 +65,11 @@ class Tuning(C.Structure):
                 ("leaf_regroup_lanes", u32), ("samples_per_job", u32), ("split_percent", u32), ("reserved", u32 * 2)]
 
 
+class RenderSettings(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("iterations", u32), ("depth", u32),
+                ("gamma", f32), ("ray_cast_epsilon", f32), ("flip_y", u32), ("seed", u32)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("width", u32), ("height", u32), ("seed", u32),
                 ("reserved", u32)]
@@ -184,6 +189,11 @@ HOST_API = [
     ("ornament_state_get_resolution", st, [vp, up, up]),
     ("ornament_state_get_current_iteration", st, [vp, fp]),
     ("ornament_path_tracer_constant_params", st, [vp, P(ConstantParams)]),
+    ("ornament_scene_save", st, [vp, P(RenderSettings), C.c_char_p]),
+    ("ornament_scene_load", st, [C.c_char_p, P(vp), P(RenderSettings), P(C.c_int)]),
+    ("ornament_image_write_pfm", st, [C.c_char_p, vp, u32, u32]),
+    ("ornament_image_read_pfm", st, [C.c_char_p, vp, sz, up, up]),
+    ("ornament_image_write_exr", st, [C.c_char_p, vp, u32, u32]),
 ]
 
 ALL_API = DEVICE_API + HOST_API

@@ -11,8 +11,12 @@
 // rounding per written operation (compiled with -ffp-contract=off), except
 // matrix inversion, which is done in double and rounded once to f32.
 #include <array>
+#include <cctype>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <memory>
 #include <new>
 #include <unordered_set>
@@ -1247,6 +1251,408 @@ orn_status ornament_path_tracer_constant_params(ornament_path_tracer *pt, orn_co
     ORN_STATE_CHECK(pt, out);
     constant_params(pt, out);
     return ORN_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scene files and images
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kSceneMagic[8] = {'O', 'R', 'N', 'S', 'C', 'E', 'N', 'E'};
+constexpr uint32_t kSceneVersion = 1;
+constexpr uint64_t kMaxArray = 1ull << 32;  // elements of one array in a scene file
+
+// Little-endian writer/reader over a byte vector (x86-64 and the GPU host
+// are little-endian, so values are copied as they are).
+struct Writer {
+    std::vector<uint8_t> b;
+    template <typename T> void put(const T &v)
+    {
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    void put_bytes(const void *p, size_t n)
+    {
+        const uint8_t *q = static_cast<const uint8_t *>(p);
+        b.insert(b.end(), q, q + n);
+    }
+};
+
+struct Reader {
+    const std::vector<uint8_t> &b;
+    size_t at = 0;
+    bool ok = true;
+    template <typename T> T get()
+    {
+        T v{};
+        if (!ok || b.size() - at < sizeof(T)) {
+            ok = false;
+            return v;
+        }
+        std::memcpy(&v, b.data() + at, sizeof(T));
+        at += sizeof(T);
+        return v;
+    }
+    // n elements of T; fails on sizes the file cannot hold
+    template <typename T> bool get_array(uint64_t n, std::vector<T> &out)
+    {
+        if (!ok || n > kMaxArray || n > (b.size() - at) / sizeof(T)) return ok = false;
+        out.resize(n);
+        if (n) std::memcpy(out.data(), b.data() + at, n * sizeof(T));
+        at += n * sizeof(T);
+        return true;
+    }
+};
+
+bool read_file(const char *path, std::vector<uint8_t> &out)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    uint8_t buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) out.insert(out.end(), buf, buf + n);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    return !err;
+}
+
+bool write_file(const char *path, const std::vector<uint8_t> &data)
+{
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(data.data(), 1, data.size(), f) == data.size();
+    return (std::fclose(f) == 0) && ok;
+}
+
+template <typename P> uint32_t index_of(const std::vector<std::unique_ptr<P>> &list, const P *p)
+{
+    for (size_t i = 0; i < list.size(); i++)
+        if (list[i].get() == p) return static_cast<uint32_t>(i);
+    return 0xffffffffu;
+}
+
+template <typename P> bool attached(const std::vector<P *> &list, const P *p)
+{
+    for (const P *q : list)
+        if (q == p) return true;
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+orn_status ornament_scene_save(const ornament_scene *s, const ornament_render_settings *settings, const char *path)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    if (!path) return ORN_ERROR_INVALID_VALUE;
+    Writer w;
+    w.put_bytes(kSceneMagic, 8);
+    w.put(kSceneVersion);
+    w.put(static_cast<uint32_t>(settings ? 1u : 0u));
+    const Camera &c = s->camera;
+    for (int i = 0; i < 3; i++) w.put(c.lookfrom[i]);
+    for (int i = 0; i < 3; i++) w.put(c.lookat[i]);
+    for (int i = 0; i < 3; i++) w.put(c.vup[i]);
+    w.put(c.aspect_ratio);
+    w.put(c.vfov);
+    w.put(c.lens_radius * 2.0f);  // aperture (camera.zig:60)
+    w.put(c.focus_dist);
+    if (settings) w.put(*settings);
+    w.put(static_cast<uint32_t>(s->textures.size()));
+    w.put(static_cast<uint32_t>(s->materials.size()));
+    w.put(static_cast<uint32_t>(s->spheres.size()));
+    w.put(static_cast<uint32_t>(s->meshes.size()));
+    w.put(static_cast<uint32_t>(s->mesh_instances.size()));
+    for (const auto &t : s->textures) {
+        w.put(t->width);
+        w.put(t->height);
+        w.put(t->num_components);
+        w.put(t->bytes_per_component);
+        w.put(static_cast<uint32_t>(t->is_hdr ? 1u : 0u));
+        w.put(t->gamma);
+        w.put(static_cast<uint64_t>(t->data.size()));
+        w.put_bytes(t->data.data(), t->data.size());
+        while (w.b.size() % 4) w.b.push_back(0);
+    }
+    for (const auto &m : s->materials) {
+        w.put(m->type);
+        w.put(m->texture ? index_of(s->textures, m->texture) : 0xffffffffu);
+        for (int i = 0; i < 3; i++) w.put(m->albedo[i]);
+        w.put(m->fuzz);
+        w.put(m->ior);
+    }
+    for (const auto &sp : s->spheres) {
+        // S(r) * T(c) in row-vector form: r on the diagonal, c in the last row
+        w.put(sp->transform[12]);
+        w.put(sp->transform[13]);
+        w.put(sp->transform[14]);
+        w.put(sp->transform[0]);
+        w.put(index_of(s->materials, sp->material));
+        w.put(static_cast<uint32_t>(attached(s->attached_spheres, sp.get())));
+    }
+    for (const auto &m : s->meshes) {
+        w.put(static_cast<uint64_t>(m->vertices.size()));
+        w.put(static_cast<uint64_t>(m->vertex_indices.size()));
+        w.put(static_cast<uint64_t>(m->normals.size()));
+        w.put(static_cast<uint64_t>(m->normal_indices.size()));
+        w.put(static_cast<uint64_t>(m->uvs.size()));
+        w.put(static_cast<uint64_t>(m->uv_indices.size()));
+        for (const Vec &v : m->vertices) w.put_bytes(v.data(), 16);
+        w.put_bytes(m->vertex_indices.data(), 4 * m->vertex_indices.size());
+        for (const Vec &v : m->normals) w.put_bytes(v.data(), 16);
+        w.put_bytes(m->normal_indices.data(), 4 * m->normal_indices.size());
+        for (const auto &uv : m->uvs) w.put_bytes(uv.data(), 8);
+        w.put_bytes(m->uv_indices.data(), 4 * m->uv_indices.size());
+        w.put_bytes(m->transform.data(), 64);
+        w.put(index_of(s->materials, m->material));
+        w.put(static_cast<uint32_t>(attached(s->attached_meshes, m.get())));
+    }
+    for (const auto &mi : s->mesh_instances) {
+        w.put(index_of(s->meshes, mi->mesh));
+        w.put_bytes(mi->transform.data(), 64);
+        w.put(index_of(s->materials, mi->material));
+        w.put(static_cast<uint32_t>(attached(s->attached_mesh_instances, mi.get())));
+    }
+    return write_file(path, w.b) ? ORN_OK : ORN_ERROR_IO;
+}
+
+orn_status ornament_scene_load(const char *path, ornament_scene **out, ornament_render_settings *settings,
+                               int *has_settings)
+{
+    if (!path || !out) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    std::vector<uint8_t> bytes;
+    if (!read_file(path, bytes)) return ORN_ERROR_IO;
+    Reader r{bytes};
+    char magic[8];
+    for (char &ch : magic) ch = static_cast<char>(r.get<uint8_t>());
+    if (!r.ok || std::memcmp(magic, kSceneMagic, 8) != 0 || r.get<uint32_t>() != kSceneVersion)
+        return ORN_ERROR_INVALID_FILE;
+    const uint32_t flags = r.get<uint32_t>();
+    float cam[13];
+    for (float &v : cam) v = r.get<float>();
+    ornament_render_settings rs{};
+    if (flags & 1u) rs = r.get<ornament_render_settings>();
+    uint32_t counts[5];
+    for (uint32_t &n : counts) n = r.get<uint32_t>();
+    if (!r.ok) return ORN_ERROR_INVALID_FILE;
+    ornament_scene *s = nullptr;
+    orn_status st = ornament_scene_init(&s);
+    if (st != ORN_OK) return st;
+    std::unique_ptr<ornament_scene> guard(s);
+    auto bad = [] { return ORN_ERROR_INVALID_FILE; };
+    std::vector<ornament_texture *> tex;
+    for (uint32_t i = 0; i < counts[0]; i++) {
+        const uint32_t wdt = r.get<uint32_t>(), hgt = r.get<uint32_t>(), comps = r.get<uint32_t>(),
+                       bpc = r.get<uint32_t>(), hdr = r.get<uint32_t>();
+        const float gamma = r.get<float>();
+        const uint64_t n = r.get<uint64_t>();
+        std::vector<uint8_t> data;
+        if (!r.get_array(n, data)) return bad();
+        while (r.at % 4 && r.ok) (void)r.get<uint8_t>();
+        ornament_texture *t = nullptr;
+        if (uint64_t(wdt) * hgt * comps * bpc != n ||
+            ornament_scene_create_texture(s, data.data(), data.size(), wdt, hgt, comps, bpc, int(hdr), gamma, &t))
+            return bad();
+        tex.push_back(t);
+    }
+    std::vector<ornament_material *> mats;
+    for (uint32_t i = 0; i < counts[1]; i++) {
+        const uint32_t type = r.get<uint32_t>(), ti = r.get<uint32_t>();
+        float albedo[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 3; k++) albedo[k] = r.get<float>();
+        const float fuzz = r.get<float>(), ior = r.get<float>();
+        if (!r.ok || type > 3u || (ti != 0xffffffffu && ti >= tex.size())) return bad();
+        ornament_texture *t = ti == 0xffffffffu ? nullptr : tex[ti];
+        ornament_material *m = nullptr;
+        switch (type) {
+        case 0: st = t ? ornament_scene_lambertian_texture(s, t, &m) : ornament_scene_lambertian(s, albedo, &m); break;
+        case 1: st = t ? ornament_scene_metal_texture(s, t, fuzz, &m) : ornament_scene_metal(s, albedo, fuzz, &m); break;
+        case 2: st = ornament_scene_dielectric(s, ior, &m); break;
+        default: st = t ? ornament_scene_diffuse_light_texture(s, t, &m) : ornament_scene_diffuse_light(s, albedo, &m);
+        }
+        if (st != ORN_OK) return st;
+        mats.push_back(m);
+    }
+    auto material = [&](uint32_t i) { return i < mats.size() ? mats[i] : nullptr; };
+    for (uint32_t i = 0; i < counts[2]; i++) {
+        float c[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        for (int k = 0; k < 3; k++) c[k] = r.get<float>();
+        const float radius = r.get<float>();
+        ornament_material *m = material(r.get<uint32_t>());
+        const uint32_t att = r.get<uint32_t>();
+        ornament_sphere *sp = nullptr;
+        if (!r.ok || !m || ornament_scene_create_sphere(s, c, radius, m, &sp)) return bad();
+        if (att) (void)ornament_scene_attach_sphere(s, sp);
+    }
+    std::vector<ornament_mesh *> meshes;
+    for (uint32_t i = 0; i < counts[3]; i++) {
+        uint64_t n[6];
+        for (uint64_t &v : n) v = r.get<uint64_t>();
+        std::vector<float> v, nn, uv;
+        std::vector<uint32_t> vi, ni, ui;
+        if (!r.get_array(n[0] * 4, v) || !r.get_array(n[1], vi) || !r.get_array(n[2] * 4, nn) ||
+            !r.get_array(n[3], ni) || !r.get_array(n[4] * 2, uv) || !r.get_array(n[5], ui))
+            return bad();
+        float t[16];
+        for (float &x : t) x = r.get<float>();
+        ornament_material *m = material(r.get<uint32_t>());
+        const uint32_t att = r.get<uint32_t>();
+        // indices must stay inside their arrays (the BVH build reads them)
+        for (uint32_t x : ni)
+            if (x >= n[2]) return bad();
+        for (uint32_t x : ui)
+            if (x >= n[4]) return bad();
+        if (n[1] % 3 || n[3] < n[1] || n[5] < n[1]) return bad();
+        ornament_mesh *mesh = nullptr;
+        if (!r.ok || !m ||
+            ornament_scene_create_mesh(s, v.data(), n[0], vi.data(), n[1], nn.data(), n[2], ni.data(), n[3],
+                                       uv.data(), n[4], ui.data(), n[5], t, m, &mesh))
+            return bad();
+        if (att) (void)ornament_scene_attach_mesh(s, mesh);
+        meshes.push_back(mesh);
+    }
+    for (uint32_t i = 0; i < counts[4]; i++) {
+        const uint32_t mi = r.get<uint32_t>();
+        float t[16];
+        for (float &x : t) x = r.get<float>();
+        ornament_material *m = material(r.get<uint32_t>());
+        const uint32_t att = r.get<uint32_t>();
+        ornament_mesh_instance *inst = nullptr;
+        if (!r.ok || !m || mi >= meshes.size() ||
+            ornament_scene_create_mesh_instance(s, meshes[mi], t, m, &inst))
+            return bad();
+        if (att) (void)ornament_scene_attach_mesh_instance(s, inst);
+    }
+    if (!r.ok || r.at != bytes.size()) return ORN_ERROR_INVALID_FILE;
+    const float from[4] = {cam[0], cam[1], cam[2], 1.0f}, at[4] = {cam[3], cam[4], cam[5], 1.0f},
+                up[4] = {cam[6], cam[7], cam[8], 0.0f};
+    st = ornament_scene_set_camera(s, from, at, up, cam[9], cam[10], cam[11], cam[12]);
+    if (st != ORN_OK) return st;
+    if (settings) *settings = rs;
+    if (has_settings) *has_settings = (flags & 1u) ? 1 : 0;
+    *out = guard.release();
+    return ORN_OK;
+}
+
+orn_status ornament_image_write_pfm(const char *path, const float *rgba, uint32_t width, uint32_t height)
+{
+    if (!path || !rgba || width == 0 || height == 0) return ORN_ERROR_INVALID_VALUE;
+    Writer w;
+    char head[64];
+    const int n = std::snprintf(head, sizeof(head), "PF\n%u %u\n-1.0\n", width, height);
+    w.put_bytes(head, static_cast<size_t>(n));
+    // a negative scale means little-endian; rows run bottom to top, as row 0
+    // of the framebuffer is the bottom of the picture
+    const size_t px = size_t(width) * height;
+    for (size_t i = 0; i < px; i++) w.put_bytes(rgba + 4 * i, 12);
+    return write_file(path, w.b) ? ORN_OK : ORN_ERROR_IO;
+}
+
+orn_status ornament_image_read_pfm(const char *path, float *rgba, size_t pixels, uint32_t *width, uint32_t *height)
+{
+    if (!path || !width || !height) return ORN_ERROR_INVALID_VALUE;
+    std::vector<uint8_t> b;
+    if (!read_file(path, b)) return ORN_ERROR_IO;
+    // three whitespace-separated header tokens after the two-byte tag
+    size_t at = 0;
+    auto token = [&](std::string &t) {
+        t.clear();
+        while (at < b.size() && std::isspace(b[at])) at++;
+        while (at < b.size() && !std::isspace(b[at])) t.push_back(static_cast<char>(b[at++]));
+        return !t.empty();
+    };
+    std::string tag, ws, hs, ss;
+    if (!token(tag) || (tag != "PF" && tag != "Pf") || !token(ws) || !token(hs) || !token(ss) || at >= b.size())
+        return ORN_ERROR_INVALID_FILE;
+    at++;  // the single whitespace byte that ends the header
+    char *e1 = nullptr, *e2 = nullptr, *e3 = nullptr;
+    const unsigned long wv = std::strtoul(ws.c_str(), &e1, 10), hv = std::strtoul(hs.c_str(), &e2, 10);
+    const double scale = std::strtod(ss.c_str(), &e3);
+    if (*e1 || *e2 || *e3 || wv == 0 || hv == 0 || wv > 65535 || hv > 65535 || scale == 0.0)
+        return ORN_ERROR_INVALID_FILE;
+    const uint32_t comps = tag == "PF" ? 3u : 1u;
+    const size_t px = size_t(wv) * hv;
+    if (b.size() - at != px * comps * 4) return ORN_ERROR_INVALID_FILE;
+    *width = static_cast<uint32_t>(wv);
+    *height = static_cast<uint32_t>(hv);
+    if (!rgba) return ORN_OK;
+    if (pixels < px) return ORN_ERROR_SIZE_MISMATCH;
+    const bool swap = scale > 0.0;  // big-endian data
+    for (size_t i = 0; i < px; i++) {
+        float c[3];
+        for (uint32_t k = 0; k < comps; k++) {
+            uint8_t q[4];
+            std::memcpy(q, b.data() + at + 4 * (i * comps + k), 4);
+            if (swap) {
+                std::swap(q[0], q[3]);
+                std::swap(q[1], q[2]);
+            }
+            std::memcpy(&c[k], q, 4);
+        }
+        rgba[4 * i] = c[0];
+        rgba[4 * i + 1] = comps == 3 ? c[1] : c[0];
+        rgba[4 * i + 2] = comps == 3 ? c[2] : c[0];
+        rgba[4 * i + 3] = 1.0f;
+    }
+    return ORN_OK;
+}
+
+orn_status ornament_image_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height)
+{
+    if (!path || !rgba || width == 0 || height == 0 || width > 0x7fffffffu || height > 0x7fffffffu)
+        return ORN_ERROR_INVALID_VALUE;
+    Writer w;
+    w.put(static_cast<uint32_t>(20000630));  // magic
+    w.put(static_cast<uint32_t>(2));         // version 2, single-part scanline
+    auto attr = [&](const char *name, const char *type, uint32_t size) {
+        w.put_bytes(name, std::strlen(name) + 1);
+        w.put_bytes(type, std::strlen(type) + 1);
+        w.put(size);
+    };
+    static const char *kChannels[4] = {"A", "B", "G", "R"};  // sorted, as the format requires
+    attr("channels", "chlist", 4 * (2 + 16) + 1);
+    for (const char *c : kChannels) {
+        w.put_bytes(c, 2);
+        w.put(static_cast<int32_t>(2));  // FLOAT
+        w.put(static_cast<uint32_t>(0)); // pLinear + reserved
+        w.put(static_cast<int32_t>(1));  // xSampling
+        w.put(static_cast<int32_t>(1));  // ySampling
+    }
+    w.put(static_cast<uint8_t>(0));
+    attr("compression", "compression", 1);
+    w.put(static_cast<uint8_t>(0));  // NO_COMPRESSION
+    const int32_t box[4] = {0, 0, static_cast<int32_t>(width) - 1, static_cast<int32_t>(height) - 1};
+    attr("dataWindow", "box2i", 16);
+    w.put_bytes(box, 16);
+    attr("displayWindow", "box2i", 16);
+    w.put_bytes(box, 16);
+    attr("lineOrder", "lineOrder", 1);
+    w.put(static_cast<uint8_t>(0));  // INCREASING_Y
+    attr("pixelAspectRatio", "float", 4);
+    w.put(1.0f);
+    attr("screenWindowCenter", "v2f", 8);
+    w.put(0.0f);
+    w.put(0.0f);
+    attr("screenWindowWidth", "float", 4);
+    w.put(1.0f);
+    w.put(static_cast<uint8_t>(0));  // end of header
+    // offset table, then one chunk per scanline: y, byte count, the
+    // channels one after another; EXR row 0 is the top of the picture
+    const uint64_t line_bytes = uint64_t(width) * 4 * 4;
+    const uint64_t table = w.b.size();
+    for (uint32_t y = 0; y < height; y++) w.put(table + 8ull * height + uint64_t(y) * (8 + line_bytes));
+    for (uint32_t y = 0; y < height; y++) {
+        w.put(static_cast<int32_t>(y));
+        w.put(static_cast<uint32_t>(line_bytes));
+        const float *row = rgba + size_t(height - 1 - y) * width * 4;
+        for (int ch : {3, 2, 1, 0})  // A, B, G, R
+            for (uint32_t x = 0; x < width; x++) w.put(row[4 * x + ch]);
+    }
+    return write_file(path, w.b) ? ORN_OK : ORN_ERROR_IO;
 }
 
 }  // extern "C"

@@ -914,6 +914,8 @@ const char *orn_status_string(orn_status s)
     case ORN_ERROR_UNSUPPORTED_TEXTURE: return "UnsupportedTexture";
     case ORN_ERROR_INVALID_HANDLE: return "InvalidHandle";
     case ORN_ERROR_SIZE_MISMATCH: return "SizeMismatch";
+    case ORN_ERROR_IO: return "IoError";
+    case ORN_ERROR_INVALID_FILE: return "InvalidFile";
     default: return hipGetErrorName(static_cast<hipError_t>(s));
     }
 }

@@ -19,6 +19,7 @@ Random choices use SplitMix64 (seeded), so every scene is reproducible:
 """
 import dataclasses
 import math
+import os
 
 import numpy as np
 
@@ -382,6 +383,102 @@ def build_scene(spec):
     lookfrom, lookat, vup, aspect, vfov, aperture, focus = spec.camera
     s.set_camera(lookfrom, lookat, vup, aspect, vfov, aperture, focus)
     return s
+
+
+def spec_settings(spec):
+    """The render settings of a spec, as stored in a scene file."""
+    return dict(width=spec.width, height=spec.height, iterations=spec.spp, depth=spec.depth,
+                gamma=spec.gamma, ray_cast_epsilon=spec.ray_cast_epsilon,
+                flip_y=1 if spec.flip_y else 0, seed=spec.seed)
+
+
+def save_spec(spec, path):
+    """Writes a spec as a scene file (ornament_scene_save of the Scene the
+    spec builds, with its render settings)."""
+    build_scene(spec).save(path, spec_settings(spec))
+
+
+_MATERIAL_KINDS = {0: "lambertian", 1: "metal", 2: "dielectric", 3: "light"}
+
+
+def load_spec(path):
+    """Reads a scene file (the ORNSCENE layout ornament_scene_save writes)
+    into a SceneSpec whose meshes are explicit arrays; building it with
+    build_scene gives the Scene ornament_scene_load gives."""
+    b = open(path, "rb").read()
+    at = [0]
+
+    def take(fmt_dtype, count):
+        dt = np.dtype(fmt_dtype)
+        n = dt.itemsize * count
+        if at[0] + n > len(b):
+            raise ValueError(f"{path}: truncated scene file")
+        a = np.frombuffer(b, dt, count, at[0])
+        at[0] += n
+        return a
+
+    if b[:8] != b"ORNSCENE":
+        raise ValueError(f"{path}: not a scene file")
+    at[0] = 8
+    version, flags = take("<u4", 2)
+    if version != 1:
+        raise ValueError(f"{path}: scene file version {version}")
+    cam = take("<f4", 13)
+    settings = None
+    if flags & 1:
+        w, h, it, depth = (int(x) for x in take("<u4", 4))
+        gamma, eps = (float(x) for x in take("<f4", 2))
+        flip, seed = (int(x) for x in take("<u4", 2))
+        settings = dict(width=w, height=h, iterations=it, depth=depth, gamma=gamma,
+                        ray_cast_epsilon=eps, flip_y=flip, seed=seed)
+    n_tex, n_mat, n_sph, n_mesh, n_inst = (int(x) for x in take("<u4", 5))
+    st = settings or dict(width=1, height=1, iterations=1, depth=1, gamma=1.0,
+                          ray_cast_epsilon=0.001, flip_y=0, seed=0)
+    camera = (tuple(float(x) for x in cam[0:3]), tuple(float(x) for x in cam[3:6]),
+              tuple(float(x) for x in cam[6:9]), float(cam[9]), float(cam[10]), float(cam[11]),
+              float(cam[12]))
+    spec = SceneSpec(os.path.basename(path), st["width"], st["height"], st["iterations"],
+                     st["depth"], camera, ray_cast_epsilon=st["ray_cast_epsilon"],
+                     gamma=st["gamma"], flip_y=bool(st["flip_y"]), seed=st["seed"])
+    for _ in range(n_tex):
+        w, h, comps, bpc, hdr = (int(x) for x in take("<u4", 5))
+        take("<f4", 1)  # gamma of the texture (not used by the kernels)
+        n = int(take("<u8", 1)[0])
+        data = take("u1", n).copy()
+        at[0] += (-at[0]) % 4
+        spec.textures.append(dict(data=data, width=w, height=h, num_components=comps,
+                                  bytes_per_component=bpc, is_hdr=bool(hdr)))
+    for _ in range(n_mat):
+        kind, tex = (int(x) for x in take("<u4", 2))
+        albedo = tuple(float(x) for x in take("<f4", 3))
+        fuzz, ior = (float(x) for x in take("<f4", 2))
+        a = ("tex", tex) if tex != 0xFFFFFFFF else albedo
+        param = fuzz if kind == 1 else ior if kind == 2 else None
+        spec.materials.append((_MATERIAL_KINDS[kind], None if kind == 2 else a, param))
+    for _ in range(n_sph):
+        c = tuple(float(x) for x in take("<f4", 3))
+        r = float(take("<f4", 1)[0])
+        m, _att = (int(x) for x in take("<u4", 2))
+        spec.spheres.append((c, r, m))
+    for _ in range(n_mesh):
+        nv, nvi, nn, nni, nuv, nui = (int(x) for x in take("<u8", 6))
+        v = take("<f4", 4 * nv).reshape(-1, 4).copy()
+        vi = take("<u4", nvi).copy()
+        nrm = take("<f4", 4 * nn).reshape(-1, 4).copy()
+        ni = take("<u4", nni).copy()
+        uv = take("<f4", 2 * nuv).reshape(-1, 2).copy()
+        ui = take("<u4", nui).copy()
+        t = take("<f4", 16).reshape(4, 4).copy()
+        m, _att = (int(x) for x in take("<u4", 2))
+        spec.meshes.append(("mesh", v, vi, nrm, ni, uv, ui, t, m))
+    for _ in range(n_inst):
+        mesh = int(take("<u4", 1)[0])
+        t = take("<f4", 16).reshape(4, 4).copy()
+        m, _att = (int(x) for x in take("<u4", 2))
+        spec.instances.append((mesh, t, m))
+    if at[0] != len(b):
+        raise ValueError(f"{path}: trailing bytes in scene file")
+    return spec, settings
 
 
 def build_scene_camera(spec):
