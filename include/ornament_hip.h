@@ -334,6 +334,91 @@ orn_status orn_batch_autotune(orn_batch *batch, const orn_constant_params *param
                               orn_tuning *chosen, float *ms);
 
 /* ------------------------------------------------------------------ */
+/* Neural radiance cache (no counterpart in the reference: an addition  */
+/* that changes the method; the plain tracer above stays the judge)     */
+/* ------------------------------------------------------------------ */
+
+/* A multiresolution hash grid (16 levels x 2 FP16 features, 2^log2_table
+ * entries per level) over the scene box feeds a 5 x 64 ReLU MLP (BF16
+ * weights and activations, FP32 accumulation) that predicts the value the
+ * reference estimator (ray_color, pathtracer.wgsl:95-161) would return for
+ * the rest of a path from a shading point. orn_nrc_render traces each
+ * sample with the reference arithmetic up to the hit of bounce
+ * terminate_bounce and, on a scattering surface, replaces the rest of the
+ * path with throughput x prediction. orn_nrc_train traces train_paths paths
+ * with their own RNG streams to full depth, records the inputs and the
+ * estimator's value at that vertex, and takes Adam steps (relative L2 loss). */
+typedef struct orn_nrc orn_nrc;
+typedef struct orn_nrc_config {
+    uint32_t log2_table;       /* default 19 */
+    uint32_t base_resolution;  /* coarsest grid, cells per axis (default 16) */
+    uint32_t max_resolution;   /* finest grid (default 2048) */
+    uint32_t terminate_bounce; /* bounce whose hit is queried (default 1; 0: primary hit) */
+    uint32_t train_paths;      /* training paths per orn_nrc_train (default 65536) */
+    uint32_t train_batch;      /* samples per Adam step, a multiple of 64 (default 16384) */
+    float learning_rate;       /* default 1e-2 */
+    float beta1, beta2, epsilon; /* default 0.9, 0.99, 1e-15 */
+    uint32_t seed;             /* initial parameters (default 1) */
+    uint32_t reserved[5];
+} orn_nrc_config;
+/* One query or training sample: the cache's inputs at a shading point and
+ * the path throughput up to it (queries) or the target value (training). */
+typedef struct orn_nrc_record {
+    float pos[3];
+    uint32_t pixel;
+    float dir[3];      /* towards the viewer: -normalize(ray direction) */
+    float roughness;   /* metal fuzz, else 0 */
+    float normal[3];   /* front-facing shading normal */
+    uint32_t type;     /* material type 0..3 */
+    float albedo[3];   /* albedo or texel; 1 for dielectrics */
+    uint32_t flags;
+    float weight[3];
+    uint32_t pad;
+} orn_nrc_record;
+typedef struct orn_nrc_info_t {
+    uint32_t levels;
+    uint32_t log2_table;
+    uint32_t resolution[16]; /* cells per axis of each level */
+    uint32_t dense[16];      /* 1: level indexed directly, 0: hashed */
+    float bounds_lo[3];      /* position -> [0,1]^3: (p - lo) * inv_extent, clamped */
+    float inv_extent[3];
+    uint32_t mlp_params;     /* 21,504: W0..W4 64 x 64 then W5 64 x 16, each [in][out] */
+    uint64_t grid_params;    /* levels x 2^log2_table x 2 */
+    uint32_t adam_steps;
+    uint32_t reserved[7];
+} orn_nrc_info_t;
+orn_status orn_nrc_default_config(orn_nrc_config *config);
+orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *config, orn_nrc **out);
+orn_status orn_nrc_destroy(orn_nrc *nrc);
+orn_status orn_nrc_get_info(orn_nrc *nrc, orn_nrc_info_t *info);
+/* One render() (params and iterations as for orn_render on the same target;
+ * the target must belong to the cache's scene) with cached termination. */
+orn_status orn_nrc_render(orn_nrc *nrc, orn_target *target, const orn_constant_params *params,
+                          uint32_t iterations);
+/* Training paths of frame `frame` (pixels and RNG streams hashed from it)
+ * and `steps` Adam steps of train_batch samples; loss (optional) returns the
+ * mean loss of the steps. */
+orn_status orn_nrc_train(orn_nrc *nrc, const orn_constant_params *params, uint32_t frame,
+                         uint32_t steps, float *loss);
+/* GPU milliseconds of the latest orn_nrc_render and orn_nrc_train. */
+orn_status orn_nrc_last_ms(orn_nrc *nrc, float *render_ms, float *train_ms);
+/* Parameters (FP32 masters; the kernels read BF16 / FP16 copies made from
+ * them). mlp: 21,504 floats; grid: [level][entry][feature]. */
+orn_status orn_nrc_get_params(orn_nrc *nrc, float *mlp, size_t mlp_len, float *grid, size_t grid_len);
+orn_status orn_nrc_set_params(orn_nrc *nrc, const float *mlp, size_t mlp_len, const float *grid,
+                              size_t grid_len);
+/* Raw predictions (3 floats per record) and, optionally, the network's 64
+ * inputs per record before BF16 rounding. */
+orn_status orn_nrc_query(orn_nrc *nrc, const orn_nrc_record *records, size_t n, float *rgb,
+                         float *inputs);
+/* Loss and gradients over host records (weight = target), without an
+ * update: mlp_grad 21,504 floats, grid_grad (optional) grid_params floats. */
+orn_status orn_nrc_gradients(orn_nrc *nrc, const orn_nrc_record *records, size_t n, float *loss,
+                             float *mlp_grad, float *grid_grad, size_t grid_len);
+/* One Adam step on host records. */
+orn_status orn_nrc_step(orn_nrc *nrc, const orn_nrc_record *records, size_t n, float *loss);
+
+/* ------------------------------------------------------------------ */
 /* Ray queries                                                         */
 /* ------------------------------------------------------------------ */
 

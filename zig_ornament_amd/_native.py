@@ -70,6 +70,19 @@ class RenderSettings(C.Structure):
                 ("gamma", f32), ("ray_cast_epsilon", f32), ("flip_y", u32), ("seed", u32)]
 
 
+class NrcConfig(C.Structure):
+    _fields_ = [("log2_table", u32), ("base_resolution", u32), ("max_resolution", u32),
+                ("terminate_bounce", u32), ("train_paths", u32), ("train_batch", u32),
+                ("learning_rate", f32), ("beta1", f32), ("beta2", f32), ("epsilon", f32),
+                ("seed", u32), ("reserved", u32 * 5)]
+
+
+class NrcInfo(C.Structure):
+    _fields_ = [("levels", u32), ("log2_table", u32), ("resolution", u32 * 16), ("dense", u32 * 16),
+                ("bounds_lo", f32 * 3), ("inv_extent", f32 * 3), ("mlp_params", u32),
+                ("grid_params", C.c_uint64), ("adam_steps", u32), ("reserved", u32 * 7)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("width", u32), ("height", u32), ("seed", u32),
                 ("reserved", u32)]
@@ -135,6 +148,18 @@ DEVICE_API = [
     ("orn_batch_set_state", st, [vp, sz, vp, vp, sz]),
     ("orn_target_autotune", st, [vp, P(ConstantParams), u32, u32, C.c_char_p, P(Tuning), fp]),
     ("orn_batch_autotune", st, [vp, P(ConstantParams), up, u32, C.c_char_p, P(Tuning), fp]),
+    ("orn_nrc_default_config", st, [P(NrcConfig)]),
+    ("orn_nrc_create", st, [vp, P(NrcConfig), P(vp)]),
+    ("orn_nrc_destroy", st, [vp]),
+    ("orn_nrc_get_info", st, [vp, P(NrcInfo)]),
+    ("orn_nrc_render", st, [vp, vp, P(ConstantParams), u32]),
+    ("orn_nrc_train", st, [vp, P(ConstantParams), u32, u32, fp]),
+    ("orn_nrc_last_ms", st, [vp, fp, fp]),
+    ("orn_nrc_get_params", st, [vp, vp, sz, vp, sz]),
+    ("orn_nrc_set_params", st, [vp, vp, sz, vp, sz]),
+    ("orn_nrc_query", st, [vp, vp, sz, vp, vp]),
+    ("orn_nrc_gradients", st, [vp, vp, sz, fp, vp, vp, sz]),
+    ("orn_nrc_step", st, [vp, vp, sz, fp]),
 ]
 
 HOST_API = [

@@ -33,13 +33,15 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
-    runtime_o = os.path.join(BUILD, "orn_runtime.o")
+    objs = []
+    for src in ("orn_runtime.hip", "orn_nrc.hip"):
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *COMMON, *DEVICE, *inc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     host_o = os.path.join(BUILD, "orn_host.o")
-    _run([HIPCC, *COMMON, *DEVICE, *inc, "-x", "hip", "-c",
-          os.path.join(CSRC, "orn_runtime.hip"), "-o", runtime_o])
     _run([HIPCC, *COMMON, *inc, "-c", os.path.join(CSRC, "orn_host.cpp"), "-o", host_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", runtime_o, host_o, "-o", tmp])
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", *objs, host_o, "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

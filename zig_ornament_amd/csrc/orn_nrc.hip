@@ -1,0 +1,1220 @@
+// orn_nrc.hip -- neural radiance cache on gfx950: multiresolution hash-grid
+// encoding, a fully fused 5 x 64 MLP on BF16 MFMA with FP32 accumulation
+// (inference, and training forward + backward), Adam, and the tracer kernels
+// that terminate paths at a chosen vertex (queries) and produce the
+// radiance samples the cache trains on.
+//
+// The cache learns the reference estimator's own value of a path from a
+// vertex on (ray_color, pathtracer.wgsl:95-161, with every quirk: sky
+// gradient, depth exhaustion returning the throughput, metal absorption):
+// a query at vertex k of a path stands for the rest of that path. Only the
+// termination changes; everything before vertex k is the reference's
+// arithmetic (the same device functions as the plain tracer).
+//
+// Network (all layers without bias; a constant-1 input feature gives the
+// first layer one):
+//   x0 = bf16(inputs[64])          inputs: 32 hash-grid features, direction,
+//                                   normal, albedo, roughness, material type
+//   x(l+1) = bf16(relu(x(l) W(l)))  l = 0..4, W(l) 64 x 64
+//   y = x5 W5                       W5 64 x 16, RGB in the first 3 outputs
+// Loss: relative L2, mean over samples and RGB of (y - t)^2 / (y^2 + 0.01)
+// with the denominator held constant. Backward rounds each layer's output
+// gradient to bf16 before its two products (FP32 accumulation), as the
+// forward rounds activations.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ornament_hip.h"
+#include "orn_handles.h"
+#include "orn_trace.h"
+
+using namespace orn;
+using namespace orn_rt;
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kIn = 64;     // network input width
+constexpr uint32_t kW = 64;      // hidden width
+constexpr uint32_t kOut = 16;    // output width (RGB + padding)
+constexpr uint32_t kHidden = 5;  // 64 x 64 layers
+constexpr uint32_t kMlpParams = kHidden * kIn * kW + kW * kOut;  // 21,504
+constexpr uint32_t kOutOffset = kHidden * kIn * kW;              // W5 in the parameter vector
+constexpr uint32_t kLevels = 16;
+constexpr uint32_t kGridFeatures = 2 * kLevels;
+constexpr uint32_t kStride = 72;  // LDS row stride in bf16 (64 + 8: rows start 4 banks apart)
+constexpr uint32_t kWtElems = kHidden * kW * kStride + kOut * kStride;  // transposed, padded weights
+constexpr uint32_t kTileRows = 64;                                       // samples per workgroup tile
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Hash grid layout and the scene box it covers.
+struct NrcGrid {
+    uint32_t log2_table;
+    uint32_t res[kLevels];    // cells per axis of each level
+    uint32_t dense[kLevels];  // 1: (res + 1)^3 <= table size, indexed directly
+    float lo[3];
+    float inv_ext[3];
+};
+
+// One cache query or training sample (80 bytes; orn_nrc_record).
+struct NrcRecord {
+    float pos[3];
+    uint32_t pixel;
+    float dir[3];
+    float roughness;
+    float normal[3];
+    uint32_t type;
+    float albedo[3];
+    uint32_t flags;
+    float weight[3];  // throughput of the path up to the query, or the training target
+    uint32_t pad;
+};
+static_assert(sizeof(NrcRecord) == 80, "record layout");
+
+// ---------------------------------------------------------------------------
+// numerics shared by every kernel (and restated by the oracle)
+// ---------------------------------------------------------------------------
+
+// float -> bf16, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ uint32_t grid_index(const NrcGrid &g, uint32_t l, uint32_t cx, uint32_t cy, uint32_t cz)
+{
+    if (g.dense[l]) {
+        const uint32_t r1 = g.res[l] + 1u;
+        return cx + r1 * (cy + r1 * cz);
+    }
+    return (cx ^ (cy * 2654435761u) ^ (cz * 805459861u)) & ((1u << g.log2_table) - 1u);
+}
+
+// scene position -> [0, 1]^3
+__device__ __forceinline__ void unit_position(const NrcGrid &g, const float pos[3], float u[3])
+{
+    for (int a = 0; a < 3; a++) u[a] = fminf(fmaxf((pos[a] - g.lo[a]) * g.inv_ext[a], 0.0f), 1.0f);
+}
+
+// The 8 corners of the cell around u at level l, with trilinear weights
+// ((wx * wy) * wz, corner bit 0 = x, bit 1 = y, bit 2 = z).
+__device__ __forceinline__ void level_corners(const NrcGrid &g, uint32_t l, const float u[3], uint32_t idx[8],
+                                              float w[8])
+{
+    uint32_t c0[3];
+    float fr[3];
+    const float r = static_cast<float>(g.res[l]);
+    for (int a = 0; a < 3; a++) {
+        const float x = u[a] * r;
+        const uint32_t c = min(static_cast<uint32_t>(floorf(x)), g.res[l] - 1u);
+        c0[a] = c;
+        fr[a] = x - static_cast<float>(c);
+    }
+    for (uint32_t c = 0; c < 8u; c++) {
+        const uint32_t bx = c & 1u, by = (c >> 1) & 1u, bz = (c >> 2) & 1u;
+        idx[c] = grid_index(g, l, c0[0] + bx, c0[1] + by, c0[2] + bz);
+        const float wx = bx ? fr[0] : 1.0f - fr[0];
+        const float wy = by ? fr[1] : 1.0f - fr[1];
+        const float wz = bz ? fr[2] : 1.0f - fr[2];
+        w[c] = (wx * wy) * wz;
+    }
+}
+
+// Two features of level l (corner order, feature += w * value)
+__device__ __forceinline__ void level_features(const NrcGrid &g, const __half2 *table, uint32_t l, const float u[3],
+                                               float &f0, float &f1)
+{
+    uint32_t idx[8];
+    float w[8];
+    level_corners(g, l, u, idx, w);
+    const __half2 *t = table + (size_t(l) << g.log2_table);
+    __half2 v[8];
+    for (int c = 0; c < 8; c++) v[c] = t[idx[c]];  // all gathers in flight together
+    f0 = 0.0f;
+    f1 = 0.0f;
+    for (int c = 0; c < 8; c++) {
+        const float2 fv = __half22float2(v[c]);
+        f0 = f0 + w[c] * fv.x;
+        f1 = f1 + w[c] * fv.y;
+    }
+}
+
+// Network inputs 32..63 of a record: direction and normal mapped to [0, 1],
+// albedo, roughness, material type one-hot, a constant 1, zeros.
+__device__ __forceinline__ float extra_input(const NrcRecord &r, uint32_t k)
+{
+    switch (k) {
+    case 32: case 33: case 34: return r.dir[k - 32] * 0.5f + 0.5f;
+    case 35: case 36: case 37: return r.normal[k - 35] * 0.5f + 0.5f;
+    case 38: case 39: case 40: return r.albedo[k - 38];
+    case 41: return r.roughness;
+    case 42: case 43: case 44: case 45: return r.type == k - 42 ? 1.0f : 0.0f;
+    case 46: return 1.0f;
+    default: return 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA tiles. v_mfma_f32_16x16x16_bf16: lane l supplies A[l % 16][4 (l / 16) + j]
+// and B[4 (l / 16) + j][l % 16] (j = 0..3) and holds D[4 (l / 16) + j][l % 16].
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 row4(const uint16_t *p)  // 4 consecutive (8-byte aligned)
+{
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    bf16x4 r;
+    r.x = static_cast<short>(v.x & 0xffffu);
+    r.y = static_cast<short>(v.x >> 16);
+    r.z = static_cast<short>(v.y & 0xffffu);
+    r.w = static_cast<short>(v.y >> 16);
+    return r;
+}
+
+__device__ __forceinline__ bf16x4 col4(const uint16_t *p)  // 4 rows of one column
+{
+    bf16x4 r;
+    r.x = static_cast<short>(p[0]);
+    r.y = static_cast<short>(p[kStride]);
+    r.z = static_cast<short>(p[2 * kStride]);
+    r.w = static_cast<short>(p[3 * kStride]);
+    return r;
+}
+
+// Forward of one layer for one 16-sample tile `it`: Y[i][n] = sum_k X[i][k]
+// W[k][n] over n_tiles 16-wide output tiles, from the transposed weights Wt
+// ([n][k] rows). With relu the result is stored as bf16 into Y; otherwise
+// the FP32 tile is returned through out (n_tiles must be 1).
+__device__ __forceinline__ void layer_forward(const uint16_t *Wt, uint32_t n_tiles, const uint16_t *X, uint16_t *Y,
+                                              uint32_t it, uint32_t lane, f32x4 *out)
+{
+    const uint32_t r = lane % 16u, q = lane / 16u;
+    for (uint32_t nt = 0; nt < n_tiles; nt++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t ks = 0; ks < 4u; ks++) {
+            const bf16x4 a = row4(Wt + (nt * 16u + r) * kStride + ks * 16u + 4u * q);
+            const bf16x4 b = row4(X + (it * 16u + r) * kStride + ks * 16u + 4u * q);
+            acc = mfma(a, b, acc);
+        }
+        // acc[j] = Y[it*16 + r][nt*16 + 4q + j]
+        if (out) {
+            *out = acc;
+        } else {
+            uint16_t h[4];
+            for (int j = 0; j < 4; j++) h[j] = to_bf16(fmaxf(acc[j], 0.0f));
+            uint2 v;
+            v.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
+            v.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+            *reinterpret_cast<uint2 *>(Y + (it * 16u + r) * kStride + nt * 16u + 4u * q) = v;
+        }
+    }
+}
+
+// Weight gradient tile (kt, nt) of one layer over the 64 samples of a tile:
+// dW[k][n] = sum_i X[i][k] dY[i][n]. Lane holds dW[kt*16 + 4q + j][nt*16 + r].
+__device__ __forceinline__ f32x4 weight_grad(const uint16_t *X, const uint16_t *dY, uint32_t kt, uint32_t nt,
+                                             uint32_t lane)
+{
+    const uint32_t r = lane % 16u, q = lane / 16u;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t is = 0; is < kTileRows / 16u; is++) {
+        const bf16x4 a = col4(X + (is * 16u + 4u * q) * kStride + kt * 16u + r);
+        const bf16x4 b = col4(dY + (is * 16u + 4u * q) * kStride + nt * 16u + r);
+        acc = mfma(a, b, acc);
+    }
+    return acc;
+}
+
+// Input gradient tile (it, kt): dX[i][k] = sum_n dY[i][n] W[k][n] over
+// n_steps 16-wide steps of n. Lane holds dX[it*16 + 4q + j][kt*16 + r].
+__device__ __forceinline__ f32x4 input_grad(const uint16_t *dY, const uint16_t *Wt, uint32_t n_steps, uint32_t it,
+                                            uint32_t kt, uint32_t lane)
+{
+    const uint32_t r = lane % 16u, q = lane / 16u;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t ns = 0; ns < n_steps; ns++) {
+        const bf16x4 a = row4(dY + (it * 16u + r) * kStride + ns * 16u + 4u * q);
+        const bf16x4 b = col4(Wt + (ns * 16u + 4u * q) * kStride + kt * 16u + r);
+        acc = mfma(a, b, acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ const uint16_t *layer_wt(const uint16_t *wt, uint32_t l) { return wt + l * kW * kStride; }
+
+// Record of tile row s: index base + s (modulo wrap when wrap > 0); rows at
+// or past `valid` hold no sample.
+__device__ __forceinline__ uint32_t tile_record(uint32_t base, uint32_t s, uint32_t wrap)
+{
+    return wrap ? (base + s) % wrap : base + s;
+}
+
+// Network inputs of a 64-sample tile into X (bf16 rows); rows without a
+// sample are zeros. 4 lanes per sample, 4 levels each; the first of them
+// also writes inputs 32..63. fp32_out (optional) receives the inputs before
+// rounding, at the record's index.
+__device__ __forceinline__ void encode_tile(const NrcGrid &g, const __half2 *table, const NrcRecord *recs,
+                                            uint32_t base, uint32_t valid, uint32_t wrap, uint16_t *X,
+                                            float *fp32_out)
+{
+    const uint32_t s = threadIdx.x / 4u, part = threadIdx.x % 4u;
+    uint16_t *row = X + s * kStride;
+    if (s >= valid) {
+        for (uint32_t k = part * 16u; k < part * 16u + 16u; k++) row[k] = 0u;
+        return;
+    }
+    const uint32_t i = tile_record(base, s, wrap);
+    const NrcRecord &rc = recs[i];
+    float u[3];
+    unit_position(g, rc.pos, u);
+    for (uint32_t l = part * 4u; l < part * 4u + 4u; l++) {
+        float f0, f1;
+        level_features(g, table, l, u, f0, f1);
+        row[2u * l] = to_bf16(f0);
+        row[2u * l + 1u] = to_bf16(f1);
+        if (fp32_out) {
+            fp32_out[size_t(i) * kIn + 2u * l] = f0;
+            fp32_out[size_t(i) * kIn + 2u * l + 1u] = f1;
+        }
+    }
+    for (uint32_t k = kGridFeatures + part * 8u; k < kGridFeatures + part * 8u + 8u; k++) {
+        const float v = extra_input(rc, k);
+        row[k] = to_bf16(v);
+        if (fp32_out) fp32_out[size_t(i) * kIn + k] = v;
+    }
+}
+
+__device__ __forceinline__ void load_weights(const uint16_t *__restrict__ wt, uint16_t *s_wt)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(wt);
+    uint4 *dst = reinterpret_cast<uint4 *>(s_wt);
+    for (uint32_t i = threadIdx.x; i < kWtElems / 8u; i += kBlock) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Inference: predictions for n records, 64 per workgroup tile (persistent).
+// mode 0: raw predictions to out_rgb[3 i]; mode 1: partial[pixel] += weight *
+// max(prediction, 0) (one query per pixel per launch).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __half2 *__restrict__ table,
+                                                           const uint16_t *__restrict__ wt,
+                                                           const NrcRecord *__restrict__ recs,
+                                                           const uint32_t *__restrict__ count_ptr, uint32_t count_max,
+                                                           int mode, float *__restrict__ out_rgb,
+                                                           float4 *__restrict__ partial, float *__restrict__ inputs)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtElems];
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[2][kTileRows * kStride];
+    const uint32_t n = min(count_ptr ? *count_ptr : count_max, count_max);
+    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    load_weights(wt, s_wt);
+    for (uint32_t tile = blockIdx.x; tile * kTileRows < n; tile += gridDim.x) {
+        const uint32_t first = tile * kTileRows;
+        __syncthreads();
+        encode_tile(g, table, recs, first, min(n - first, kTileRows), 0u, s_x[0], inputs);
+        __syncthreads();
+        uint32_t cur = 0;
+        for (uint32_t l = 0; l < kHidden; l++) {
+            layer_forward(layer_wt(s_wt, l), kW / 16u, s_x[cur], s_x[cur ^ 1u], wave, lane, nullptr);
+            cur ^= 1u;
+            __syncthreads();
+        }
+        f32x4 y;
+        layer_forward(layer_wt(s_wt, kHidden), 1u, s_x[cur], nullptr, wave, lane, &y);
+        // lanes 0..15 hold channels 0..3 of sample wave*16 + lane
+        const uint32_t i = first + wave * 16u + lane;
+        if (lane < 16u && i < n) {
+            if (mode == 0) {
+                out_rgb[3u * i] = y[0];
+                out_rgb[3u * i + 1u] = y[1];
+                out_rgb[3u * i + 2u] = y[2];
+            } else {
+                const NrcRecord &rc = recs[i];
+                float4 &pv = partial[rc.pixel];
+                pv.x = pv.x + rc.weight[0] * fmaxf(y[0], 0.0f);
+                pv.y = pv.y + rc.weight[1] * fmaxf(y[1], 0.0f);
+                pv.z = pv.z + rc.weight[2] * fmaxf(y[2], 0.0f);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training: forward + backward of one 64-sample tile per workgroup over a
+// batch of `batch` samples, records first, first + 1, ... taken modulo the
+// record count (a frame with fewer records than a step needs reuses them;
+// none: an empty step). Writes the tile's weight gradient to
+// partials[blockIdx.x], adds the grid gradient with atomics and the tile's
+// loss to loss_sum.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __half2 *__restrict__ table,
+                                                           const uint16_t *__restrict__ wt,
+                                                           const NrcRecord *__restrict__ recs,
+                                                           const uint32_t *__restrict__ count_ptr, uint32_t count_max,
+                                                           uint32_t first, uint32_t batch,
+                                                           float *__restrict__ partials, float *__restrict__ grid_grad,
+                                                           float *__restrict__ loss_sum)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtElems];
+    __shared__ __attribute__((aligned(16))) uint16_t s_a[kHidden + 1u][kTileRows * kStride];
+    __shared__ __attribute__((aligned(16))) uint16_t s_dy[2][kTileRows * kStride];
+    __shared__ float s_dx0[kTileRows * kGridFeatures];
+    const uint32_t total = min(count_ptr ? *count_ptr : count_max, count_max);
+    const uint32_t n_batch = total ? batch : 0u;
+    const uint32_t tile_rel = blockIdx.x * kTileRows;
+    const uint32_t valid = n_batch > tile_rel ? min(n_batch - tile_rel, kTileRows) : 0u;
+    const uint32_t tile_base = first + tile_rel;
+    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    const uint32_t r = lane % 16u, q = lane / 16u;
+    float *part = partials + size_t(blockIdx.x) * kMlpParams;
+    load_weights(wt, s_wt);
+    encode_tile(g, table, recs, tile_base, valid, total, s_a[0], nullptr);
+    __syncthreads();
+    for (uint32_t l = 0; l < kHidden; l++) {
+        layer_forward(layer_wt(s_wt, l), kW / 16u, s_a[l], s_a[l + 1u], wave, lane, nullptr);
+        __syncthreads();
+    }
+    // output layer and the loss gradient, row by row into s_dy[0] (16 columns)
+    f32x4 y;
+    layer_forward(layer_wt(s_wt, kHidden), 1u, s_a[kHidden], nullptr, wave, lane, &y);
+    {
+        const uint32_t s = wave * 16u + lane;
+        float loss = 0.0f;
+        if (lane < 16u) {
+            const uint32_t i = s < valid ? tile_record(tile_base, s, total) : 0u;
+            uint16_t *row = s_dy[0] + s * kStride;
+            const float scale = n_batch ? 1.0f / (3.0f * static_cast<float>(n_batch)) : 0.0f;
+            for (uint32_t c = 0; c < kOut; c++) {
+                float gr = 0.0f;
+                if (c < 3u && s < valid) {
+                    const float t = recs[i].weight[c];
+                    const float d = y[c] - t;
+                    const float den = y[c] * y[c] + 0.01f;
+                    loss = loss + (d * d / den) * scale;
+                    gr = (2.0f * d / den) * scale;
+                }
+                row[c] = to_bf16(gr);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+        if (lane == 0u && loss != 0.0f) atomicAdd(loss_sum, loss);
+    }
+    __syncthreads();
+    // backward, output layer first; dY of layer l is in s_dy[b]
+    uint32_t b = 0;
+    for (int l = static_cast<int>(kHidden); l >= 0; l--) {
+        const bool out_layer = l == static_cast<int>(kHidden);
+        const uint32_t n_tiles = out_layer ? kOut / 16u : kW / 16u;
+        const uint16_t *X = s_a[l];
+        const uint16_t *dY = s_dy[b];
+        const uint16_t *Wt = layer_wt(s_wt, static_cast<uint32_t>(l));
+        // weight gradient: wave w takes k-tile w, all n-tiles
+        for (uint32_t nt = 0; nt < n_tiles; nt++) {
+            const f32x4 d = weight_grad(X, dY, wave, nt, lane);
+            for (int j = 0; j < 4; j++) {
+                const uint32_t k = wave * 16u + 4u * q + static_cast<uint32_t>(j), nn = nt * 16u + r;
+                part[out_layer ? kOutOffset + k * kOut + nn : static_cast<uint32_t>(l) * kIn * kW + k * kW + nn] = d[j];
+            }
+        }
+        // input gradient: wave w takes sample tile w, all 4 k-tiles
+        for (uint32_t kt = 0; kt < kIn / 16u; kt++) {
+            const f32x4 d = input_grad(dY, Wt, n_tiles, wave, kt, lane);
+            for (int j = 0; j < 4; j++) {
+                const uint32_t s = wave * 16u + 4u * q + static_cast<uint32_t>(j), k = kt * 16u + r;
+                if (l > 0) {
+                    const float m = from_bf16(X[s * kStride + k]) > 0.0f ? d[j] : 0.0f;  // relu of layer l-1
+                    s_dy[b ^ 1u][s * kStride + k] = to_bf16(m);
+                } else if (k < kGridFeatures) {
+                    s_dx0[s * kGridFeatures + k] = d[j];
+                }
+            }
+        }
+        b ^= 1u;
+        __syncthreads();
+    }
+    // grid gradient: the 8 corners of every level of every sample
+    {
+        const uint32_t s = threadIdx.x / 4u, part4 = threadIdx.x % 4u;
+        if (s < valid) {
+            const uint32_t i = tile_record(tile_base, s, total);
+            float u[3];
+            unit_position(g, recs[i].pos, u);
+            for (uint32_t l = part4 * 4u; l < part4 * 4u + 4u; l++) {
+                uint32_t idx[8];
+                float w[8];
+                level_corners(g, l, u, idx, w);
+                const float d0 = s_dx0[s * kGridFeatures + 2u * l], d1 = s_dx0[s * kGridFeatures + 2u * l + 1u];
+                float *gg = grid_grad + (size_t(l) << g.log2_table) * 2u;
+                for (int c = 0; c < 8; c++) {
+                    if (d0 != 0.0f) atomicAdd(gg + 2u * idx[c], w[c] * d0);
+                    if (d1 != 0.0f) atomicAdd(gg + 2u * idx[c] + 1u, w[c] * d1);
+                }
+            }
+        }
+    }
+}
+
+// dW = sum over workgroup partials (partials[wg][j])
+__global__ void nrc_reduce_kernel(const float *__restrict__ partials, uint32_t groups, float *__restrict__ grad)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kMlpParams) return;
+    float s = 0.0f;
+    for (uint32_t g = 0; g < groups; g++) s = s + partials[size_t(g) * kMlpParams + j];
+    grad[j] = s;
+}
+
+struct AdamStep {
+    float lr, beta1, beta2, eps;
+    float c1, c2;  // 1 - beta1^t, 1 - beta2^t
+};
+
+__device__ __forceinline__ float adam(float p, float g, float &m, float &v, const AdamStep &a)
+{
+    m = a.beta1 * m + (1.0f - a.beta1) * g;
+    v = a.beta2 * v + (1.0f - a.beta2) * (g * g);
+    const float mh = m / a.c1, vh = v / a.c2;
+    return p - a.lr * mh / (sqrtf(vh) + a.eps);
+}
+
+// MLP parameters: master (layout [l][k][n], W5 [k][n]) and the transposed,
+// padded bf16 copy the kernels read.
+__global__ void nrc_adam_mlp_kernel(AdamStep a, float *__restrict__ w, const float *__restrict__ grad,
+                                    float *__restrict__ m, float *__restrict__ v, uint16_t *__restrict__ wt, int update)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kMlpParams) return;
+    float p = w[j];
+    if (update) {
+        float mj = m[j], vj = v[j];
+        p = adam(p, grad[j], mj, vj, a);
+        m[j] = mj;
+        v[j] = vj;
+        w[j] = p;
+    }
+    uint32_t dst;
+    if (j < kOutOffset) {
+        const uint32_t l = j / (kIn * kW), k = (j / kW) % kIn, n = j % kW;
+        dst = l * kW * kStride + n * kStride + k;
+    } else {
+        const uint32_t k = (j - kOutOffset) / kOut, n = (j - kOutOffset) % kOut;
+        dst = kHidden * kW * kStride + n * kStride + k;
+    }
+    wt[dst] = to_bf16(p);
+}
+
+// Grid parameters: master fp32, fp16 copy; the gradient is cleared.
+__global__ void nrc_adam_grid_kernel(AdamStep a, float *__restrict__ w, float *__restrict__ grad,
+                                     float *__restrict__ m, float *__restrict__ v, __half *__restrict__ half_w,
+                                     uint64_t n, int update)
+{
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float p = w[j];
+    if (update) {
+        float mj = m[j], vj = v[j];
+        p = adam(p, grad[j], mj, vj, a);
+        m[j] = mj;
+        v[j] = vj;
+        w[j] = p;
+        grad[j] = 0.0f;
+    }
+    half_w[j] = __float2half_rn(p);
+}
+
+__global__ void nrc_zero_kernel(float *a, uint64_t n)
+{
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < n) a[j] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Tracer side
+// ---------------------------------------------------------------------------
+
+// pathtracer.wgsl:107-109
+__device__ __forceinline__ f3 sky(const f3 &d)
+{
+    const f3 ud = normalize(d);
+    const float tt = 0.5f * (ud.y + 1.0f);
+    const float a = (1.0f - tt) * 1.0f;
+    return mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f);
+}
+
+// camera.wgsl:12-23 with the (x + xi) / (W - 1) jitter of pathtracer.wgsl:64-79
+__device__ __forceinline__ Ray camera_ray(const orn_constant_params &p, uint32_t x, uint32_t y, uint32_t &rng)
+{
+    const orn_camera &cam = p.camera;
+    const float u = (static_cast<float>(x) + rng_f32(rng)) / static_cast<float>(p.width - 1u);
+    const float v = (static_cast<float>(y) + rng_f32(rng)) / static_cast<float>(p.height - 1u);
+    const f3 disk = random_in_unit_disk(rng);
+    const f3 rd = cam.lens_radius * disk;
+    const f3 offset = rd.x * mk(cam.u[0], cam.u[1], cam.u[2]) + rd.y * mk(cam.v[0], cam.v[1], cam.v[2]);
+    const f3 origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+    Ray ray;
+    ray.o = origin + offset;
+    ray.d = ((mk(cam.lower_left_corner[0], cam.lower_left_corner[1], cam.lower_left_corner[2]) +
+              u * mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2])) +
+             v * mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]) - origin) -
+            offset;
+    return ray;
+}
+
+// Cache inputs at a shading point
+template <uint32_t F>
+__device__ __forceinline__ void fill_record(const DevScene &sc, const Surface &s, const Ray &ray, NrcRecord &rc)
+{
+    rc.pos[0] = s.p.x;
+    rc.pos[1] = s.p.y;
+    rc.pos[2] = s.p.z;
+    const f3 ud = normalize(ray.d);
+    rc.dir[0] = -ud.x;
+    rc.dir[1] = -ud.y;
+    rc.dir[2] = -ud.z;
+    rc.normal[0] = s.normal.x;
+    rc.normal[1] = s.normal.y;
+    rc.normal[2] = s.normal.z;
+    rc.type = s.m.type;
+    rc.roughness = s.m.type == 1u ? s.m.fuzz : 0.0f;
+    const f3 alb = s.m.type == 2u ? mk(1.0f, 1.0f, 1.0f)
+                                  : (s.textured ? sample_texture(sc, s.m.texture, s.tu, s.tv) : s.m.albedo);
+    rc.albedo[0] = alb.x;
+    rc.albedo[1] = alb.y;
+    rc.albedo[2] = alb.z;
+    rc.flags = 0u;
+    rc.pad = 0u;
+}
+
+// The reference's bounce loop (pathtracer.wgsl:99-160) from bounce b0 with
+// throughput thr: returns the sample's value.
+template <uint32_t F>
+__device__ __forceinline__ f3 continue_path(const DevScene &sc, const orn_constant_params &p, Ray ray, f3 thr,
+                                            uint32_t b0, uint32_t &rng, uint32_t *stack)
+{
+    for (uint32_t b = b0; b < p.depth; b++) {
+        Hit h{};
+        if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) return thr * sky(ray.d);
+        Ray next;
+        f3 w;
+        if (shade<F>(sc, p.textures_count, 0u, ray, h, rng, next, w)) {
+            ray = next;
+            thr = thr * w;
+        } else {
+            return thr * w;
+        }
+    }
+    return thr;  // depth exhausted: the throughput
+}
+
+// One sample per pixel: the reference path up to the hit at bounce k, where a
+// scattering surface becomes a cache query (the sample's value is then
+// throughput x prediction, added by the inference kernel); paths that end
+// earlier (sky, emitter, absorption, depth) keep their exact value.
+template <uint32_t F>
+__global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
+                                                           uint32_t *__restrict__ rng_buf,
+                                                           float4 *__restrict__ partial,
+                                                           NrcRecord *__restrict__ queries,
+                                                           uint32_t *__restrict__ query_count)
+{
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t *stack = lds_stack + threadIdx.x;
+    const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= p.width * p.height) return;
+    const uint32_t x = pix % p.width, y = pix / p.width;
+    uint32_t rng = rng_buf[pix];
+    Ray ray = camera_ray(p, x, y, rng);
+    f3 thr = mk(1.0f, 1.0f, 1.0f);
+    f3 color = thr;
+    bool done = false;
+    for (uint32_t b = 0; b < p.depth && !done; b++) {
+        Hit h{};
+        if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
+            color = thr * sky(ray.d);
+            done = true;
+            break;
+        }
+        const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
+        if (b == k && s.m.type <= 2u) {
+            NrcRecord rc;
+            fill_record<F>(sc, s, ray, rc);
+            rc.pixel = pix;
+            rc.weight[0] = thr.x;
+            rc.weight[1] = thr.y;
+            rc.weight[2] = thr.z;
+            queries[atomicAdd(query_count, 1u)] = rc;
+            color = mk(0.0f, 0.0f, 0.0f);
+            done = true;
+            break;
+        }
+        Ray next;
+        f3 w;
+        if (scatter<F>(sc, s, ray, rng, next, w)) {
+            ray = next;
+            thr = thr * w;
+        } else {
+            color = thr * w;
+            done = true;
+        }
+    }
+    if (!done) color = thr;
+    partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+    rng_buf[pix] = rng;
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x)  // lowbias32 (Wellons)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Training samples: paths from random pixels with their own RNG streams;
+// at the hit of bounce k on a scattering surface the inputs are recorded and
+// the path continues with the reference estimator; the record's target is
+// the value of the path from that vertex on (throughput 1 at the vertex).
+template <uint32_t F>
+__global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
+                                                                 uint32_t n_paths, uint32_t frame,
+                                                                 NrcRecord *__restrict__ recs,
+                                                                 uint32_t *__restrict__ rec_count)
+{
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t *stack = lds_stack + threadIdx.x;
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n_paths) return;
+    const uint32_t npx = p.width * p.height;
+    const uint32_t pix = mix32(j ^ mix32(frame * 0x9e3779b9u + 0x632be5abu)) % npx;
+    uint32_t rng = mix32(pix + mix32(frame ^ 0x85ebca6bu) + j * 0xc2b2ae35u);
+    Ray ray = camera_ray(p, pix % p.width, pix / p.width, rng);
+    for (uint32_t b = 0; b <= k && b < p.depth; b++) {
+        Hit h{};
+        if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) return;
+        const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
+        Ray next;
+        f3 w;
+        if (b == k) {
+            if (s.m.type > 2u) return;  // emitters are exact, never queried
+            NrcRecord rc;
+            fill_record<F>(sc, s, ray, rc);
+            rc.pixel = pix;
+            f3 v;
+            if (scatter<F>(sc, s, ray, rng, next, w)) v = continue_path<F>(sc, p, next, w, b + 1u, rng, stack);
+            else v = w;
+            rc.weight[0] = v.x;
+            rc.weight[1] = v.y;
+            rc.weight[2] = v.z;
+            recs[atomicAdd(rec_count, 1u)] = rc;
+            return;
+        }
+        if (!scatter<F>(sc, s, ray, rng, next, w)) return;
+        ray = next;
+    }
+}
+
+// render() bookkeeping of one sample per pixel (pathtracer.wgsl:64-79):
+// current_iteration + 1, then overwrite or add.
+__global__ void nrc_accumulate_kernel(const float4 *__restrict__ partial, float4 *__restrict__ acc, uint32_t n,
+                                      float cur)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = partial[i];
+    if (cur > 1.0f) {
+        float4 a = acc[i];
+        a.x = a.x + c.x;
+        a.y = a.y + c.y;
+        a.z = a.z + c.z;
+        a.w = a.w + 1.0f;
+        acc[i] = a;
+    } else {
+        acc[i] = make_float4(c.x, c.y, c.z, 1.0f);
+    }
+}
+
+// pathtracer.wgsl:81-93
+__global__ void nrc_post_kernel(const float4 *__restrict__ acc, float4 *__restrict__ fb, uint32_t width,
+                                uint32_t height, float cur, float inv_gamma, uint32_t flip_y)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= width * height) return;
+    const float4 a = acc[i];
+    float4 o;
+    o.x = pow_f(fminf(fmaxf(a.x / cur, 0.0f), 1.0f), inv_gamma);
+    o.y = pow_f(fminf(fmaxf(a.y / cur, 0.0f), 1.0f), inv_gamma);
+    o.z = pow_f(fminf(fmaxf(a.z / cur, 0.0f), 1.0f), inv_gamma);
+    o.w = fminf(fmaxf(a.w / cur, 0.0f), 1.0f);
+    const uint32_t x = i % width, y = i / width;
+    fb[flip_y < 1u ? i : width * (height - y - 1u) + x] = o;
+}
+
+__global__ void nrc_reset_count_kernel(uint32_t *c)
+{
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host
+// ===========================================================================
+
+struct orn_nrc {
+    orn_device_scene *scene;
+    orn_nrc_config cfg;
+    NrcGrid grid;
+    uint64_t grid_params;  // levels x table x 2
+    float *mlp, *mlp_m, *mlp_v, *mlp_grad;
+    uint16_t *mlp_wt;
+    float *grid_w, *grid_m, *grid_v, *grid_grad;
+    __half *grid_half;
+    float *partials;  // per training workgroup weight gradients
+    uint32_t partial_groups;
+    NrcRecord *train_recs;
+    uint32_t train_cap;
+    NrcRecord *queries;
+    float4 *partial;
+    uint32_t query_cap;  // pixels the query and partial buffers hold
+    uint32_t *counters;  // [0] queries, [1] training records
+    float *loss;         // [0] loss sum of the current step
+    uint32_t step;       // Adam steps taken
+    hipEvent_t ev[4];    // render start/stop, train start/stop
+    bool rendered, trained;
+};
+
+namespace {
+
+template <uint32_t F>
+void launch_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, uint32_t *rng, hipStream_t st,
+                  size_t lds)
+{
+    const uint32_t px = p.width * p.height;
+    nrc_paths_kernel<F><<<blocks_for(px, kBlock), kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, n->partial,
+                                                                     n->queries, n->counters);
+}
+
+template <uint32_t F>
+void launch_train_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, uint32_t frame, hipStream_t st,
+                        size_t lds)
+{
+    nrc_train_paths_kernel<F><<<blocks_for(n->cfg.train_paths, kBlock), kBlock, lds, st>>>(
+        ds, p, n->cfg.terminate_bounce, n->cfg.train_paths, frame, n->train_recs, n->counters + 1);
+}
+
+#define ORN_NRC_DISPATCH(fn, features, ...)        \
+    switch ((features) & 7u) {                     \
+    case 0: fn<0>(__VA_ARGS__); break;             \
+    case 1: fn<1>(__VA_ARGS__); break;             \
+    case 2: fn<2>(__VA_ARGS__); break;             \
+    case 3: fn<3>(__VA_ARGS__); break;             \
+    case 4: fn<4>(__VA_ARGS__); break;             \
+    case 5: fn<5>(__VA_ARGS__); break;             \
+    case 6: fn<6>(__VA_ARGS__); break;             \
+    default: fn<7>(__VA_ARGS__); break;            \
+    }
+
+void free_nrc(orn_nrc *n)
+{
+    (void)hipSetDevice(n->scene->ctx->device);
+    (void)hipStreamSynchronize(n->scene->ctx->stream);
+    void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
+                    n->grid_grad, n->grid_half, n->partials, n->train_recs, n->queries, n->partial, n->counters,
+                    n->loss};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : n->ev)
+        if (e) (void)hipEventDestroy(e);
+    orn_device_scene *s = n->scene;
+    delete n;
+    release_scene(s);
+}
+
+AdamStep adam_step(const orn_nrc *n)
+{
+    AdamStep a;
+    a.lr = n->cfg.learning_rate;
+    a.beta1 = n->cfg.beta1;
+    a.beta2 = n->cfg.beta2;
+    a.eps = n->cfg.epsilon;
+    a.c1 = static_cast<float>(1.0 - std::pow(double(n->cfg.beta1), double(n->step)));
+    a.c2 = static_cast<float>(1.0 - std::pow(double(n->cfg.beta2), double(n->step)));
+    return a;
+}
+
+// Working copies from the masters (update = 0) or an Adam step (update = 1).
+hipError_t apply_params(orn_nrc *n, int update, hipStream_t st)
+{
+    const AdamStep a = adam_step(n);
+    nrc_adam_mlp_kernel<<<blocks_for(kMlpParams, 256), 256, 0, st>>>(a, n->mlp, n->mlp_grad, n->mlp_m, n->mlp_v,
+                                                                     n->mlp_wt, update);
+    const uint64_t g = n->grid_params;
+    nrc_adam_grid_kernel<<<static_cast<uint32_t>((g + 255) / 256), 256, 0, st>>>(a, n->grid_w, n->grid_grad, n->grid_m,
+                                                                                n->grid_v, n->grid_half, g, update);
+    return hipGetLastError();
+}
+
+// Forward + backward over records [first, first + batch) of buf (count from
+// the device counter or count_max), weight gradient reduced into mlp_grad.
+hipError_t gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr, uint32_t count_max, uint32_t first,
+                     uint32_t batch, hipStream_t st)
+{
+    const uint32_t groups = blocks_for(batch, kTileRows);
+    if (groups > n->partial_groups) return hipErrorInvalidValue;
+    nrc_train_kernel<<<groups, kBlock, 0, st>>>(n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt,
+                                                buf, count_ptr, count_max, first, batch, n->partials, n->grid_grad,
+                                                n->loss);
+    nrc_reduce_kernel<<<blocks_for(kMlpParams, 256), 256, 0, st>>>(n->partials, groups, n->mlp_grad);
+    return hipGetLastError();
+}
+
+// SplitMix64 for the initial parameters (reproducible per seed)
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float uniform() { return static_cast<float>((next() >> 40) * (1.0 / 16777216.0)); }  // [0, 1)
+};
+
+}  // namespace
+
+extern "C" {
+
+orn_status orn_nrc_default_config(orn_nrc_config *c)
+{
+    if (!c) return ORN_ERROR_INVALID_VALUE;
+    std::memset(c, 0, sizeof(*c));
+    c->log2_table = 19;
+    c->base_resolution = 16;
+    c->max_resolution = 2048;
+    c->terminate_bounce = 1;
+    c->train_paths = 65536;
+    c->train_batch = 16384;
+    c->learning_rate = 1e-2f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.99f;
+    c->epsilon = 1e-15f;
+    c->seed = 1;
+    return ORN_OK;
+}
+
+orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, orn_nrc **out)
+{
+    if (!scene) return ORN_ERROR_INVALID_HANDLE;
+    if (!cfg || !out) return ORN_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    if (cfg->log2_table < 8 || cfg->log2_table > 24 || cfg->base_resolution < 2 ||
+        cfg->max_resolution < cfg->base_resolution || cfg->max_resolution > (1u << 20) || cfg->train_batch == 0 ||
+        cfg->train_batch % kTileRows || cfg->train_batch > (1u << 24) || cfg->train_paths > (1u << 26) ||
+        !(cfg->learning_rate > 0.0f) || !(cfg->beta1 >= 0.0f && cfg->beta1 < 1.0f) ||
+        !(cfg->beta2 >= 0.0f && cfg->beta2 < 1.0f) || !(cfg->epsilon >= 0.0f))
+        return ORN_ERROR_INVALID_VALUE;
+    ORN_TRY(hipSetDevice(scene->ctx->device));
+    orn_nrc *n = new (std::nothrow) orn_nrc();
+    if (!n) return ORN_ERROR_OUT_OF_MEMORY;
+    n->scene = scene;
+    scene->refs++;
+    n->cfg = *cfg;
+    NrcGrid &g = n->grid;
+    g.log2_table = cfg->log2_table;
+    const double growth = std::exp((std::log(double(cfg->max_resolution)) - std::log(double(cfg->base_resolution))) /
+                                   double(kLevels - 1));
+    const uint64_t table = 1ull << cfg->log2_table;
+    for (uint32_t l = 0; l < kLevels; l++) {
+        g.res[l] = std::max<uint32_t>(1u, static_cast<uint32_t>(std::floor(cfg->base_resolution * std::pow(growth, l))));
+        const uint64_t r1 = g.res[l] + 1ull;
+        g.dense[l] = r1 * r1 * r1 <= table ? 1u : 0u;
+    }
+    for (int a = 0; a < 3; a++) {
+        float lo = scene->bounds_lo[a], hi = scene->bounds_hi[a];
+        if (!std::isfinite(lo) || !std::isfinite(hi) || !(hi > lo)) {  // a flat or empty box
+            lo = std::isfinite(lo) ? lo : 0.0f;
+            hi = lo + 1.0f;
+        }
+        g.lo[a] = lo;
+        g.inv_ext[a] = 1.0f / (hi - lo);
+    }
+    n->grid_params = uint64_t(kLevels) * table * 2u;
+    n->train_cap = std::max(cfg->train_paths, cfg->train_batch);
+    n->partial_groups = blocks_for(std::max(cfg->train_batch, n->train_cap), kTileRows);
+    auto fail = [&](hipError_t e) {
+        free_nrc(n);
+        return from_hip(e);
+    };
+    hipError_t e;
+    const size_t gp = size_t(n->grid_params);
+    if ((e = hipMalloc(&n->mlp, kMlpParams * 4)) || (e = hipMalloc(&n->mlp_m, kMlpParams * 4)) ||
+        (e = hipMalloc(&n->mlp_v, kMlpParams * 4)) || (e = hipMalloc(&n->mlp_grad, kMlpParams * 4)) ||
+        (e = hipMalloc(&n->mlp_wt, kWtElems * 2)) || (e = hipMalloc(&n->grid_w, gp * 4)) ||
+        (e = hipMalloc(&n->grid_m, gp * 4)) || (e = hipMalloc(&n->grid_v, gp * 4)) ||
+        (e = hipMalloc(&n->grid_grad, gp * 4)) || (e = hipMalloc(&n->grid_half, gp * 2)) ||
+        (e = hipMalloc(&n->partials, size_t(n->partial_groups) * kMlpParams * 4)) ||
+        (e = hipMalloc(&n->train_recs, size_t(n->train_cap) * sizeof(NrcRecord))) ||
+        (e = hipMalloc(&n->counters, 16)) || (e = hipMalloc(&n->loss, 16)))
+        return fail(e);
+    for (hipEvent_t &ev : n->ev)
+        if ((e = hipEventCreate(&ev))) return fail(e);
+    // initial parameters: MLP uniform in +-sqrt(6 / (fan_in + fan_out)),
+    // grid uniform in +-1e-4
+    SplitMix rng{cfg->seed};
+    std::vector<float> w(kMlpParams), gw(gp);
+    for (uint32_t j = 0; j < kMlpParams; j++) {
+        const float fan = j < kOutOffset ? float(kIn + kW) : float(kW + kOut);
+        w[j] = (rng.uniform() * 2.0f - 1.0f) * std::sqrt(6.0f / fan);
+    }
+    for (size_t j = 0; j < gp; j++) gw[j] = (rng.uniform() * 2.0f - 1.0f) * 1e-4f;
+    hipStream_t st = scene->ctx->stream;
+    const uint32_t zero_blocks = static_cast<uint32_t>((gp + 255) / 256);
+    if ((e = hipMemcpyAsync(n->mlp, w.data(), kMlpParams * 4, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(n->grid_w, gw.data(), gp * 4, hipMemcpyHostToDevice, st)))
+        return fail(e);
+    nrc_zero_kernel<<<blocks_for(kMlpParams, 256), 256, 0, st>>>(n->mlp_m, kMlpParams);
+    nrc_zero_kernel<<<blocks_for(kMlpParams, 256), 256, 0, st>>>(n->mlp_v, kMlpParams);
+    nrc_zero_kernel<<<zero_blocks, 256, 0, st>>>(n->grid_m, gp);
+    nrc_zero_kernel<<<zero_blocks, 256, 0, st>>>(n->grid_v, gp);
+    nrc_zero_kernel<<<zero_blocks, 256, 0, st>>>(n->grid_grad, gp);
+    nrc_zero_kernel<<<blocks_for(kWtElems / 2u, 256), 256, 0, st>>>(reinterpret_cast<float *>(n->mlp_wt),
+                                                                    kWtElems / 2u);  // row padding stays 0
+    if ((e = apply_params(n, 0, st)) || (e = hipStreamSynchronize(st))) return fail(e);
+    *out = n;
+    return ORN_OK;
+}
+
+orn_status orn_nrc_destroy(orn_nrc *n)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    free_nrc(n);
+    return ORN_OK;
+}
+
+orn_status orn_nrc_get_info(orn_nrc *n, orn_nrc_info_t *info)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if (!info) return ORN_ERROR_INVALID_VALUE;
+    std::memset(info, 0, sizeof(*info));
+    info->levels = kLevels;
+    info->log2_table = n->grid.log2_table;
+    for (uint32_t l = 0; l < kLevels; l++) {
+        info->resolution[l] = n->grid.res[l];
+        info->dense[l] = n->grid.dense[l];
+    }
+    for (int a = 0; a < 3; a++) {
+        info->bounds_lo[a] = n->grid.lo[a];
+        info->inv_extent[a] = n->grid.inv_ext[a];
+    }
+    info->mlp_params = kMlpParams;
+    info->grid_params = n->grid_params;
+    info->adam_steps = n->step;
+    return ORN_OK;
+}
+
+orn_status orn_nrc_get_params(orn_nrc *n, float *mlp, size_t mlp_len, float *grid, size_t grid_len)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if ((mlp && mlp_len < kMlpParams) || (grid && grid_len < n->grid_params)) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
+    if (mlp) ORN_TRY(hipMemcpyAsync(mlp, n->mlp, kMlpParams * 4, hipMemcpyDeviceToHost, st));
+    if (grid) ORN_TRY(hipMemcpyAsync(grid, n->grid_w, size_t(n->grid_params) * 4, hipMemcpyDeviceToHost, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+orn_status orn_nrc_set_params(orn_nrc *n, const float *mlp, size_t mlp_len, const float *grid, size_t grid_len)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if ((mlp && mlp_len != kMlpParams) || (grid && grid_len != n->grid_params)) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
+    if (mlp) ORN_TRY(hipMemcpyAsync(n->mlp, mlp, kMlpParams * 4, hipMemcpyHostToDevice, st));
+    if (grid) ORN_TRY(hipMemcpyAsync(n->grid_w, grid, size_t(n->grid_params) * 4, hipMemcpyHostToDevice, st));
+    ORN_TRY(apply_params(n, 0, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+orn_status orn_nrc_query(orn_nrc *n, const orn_nrc_record *records, size_t count, float *rgb, float *inputs)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if (!records || !rgb || count == 0 || count > (1u << 24)) return ORN_ERROR_INVALID_VALUE;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
+    NrcRecord *d_rec = nullptr;
+    float *d_out = nullptr, *d_in = nullptr;
+    hipError_t e = hipMalloc(&d_rec, count * sizeof(NrcRecord));
+    if (!e) e = hipMalloc(&d_out, count * 12);
+    if (!e && inputs) e = hipMalloc(&d_in, count * kIn * 4);
+    if (!e) e = hipMemcpyAsync(d_rec, records, count * sizeof(NrcRecord), hipMemcpyHostToDevice, st);
+    if (!e) {
+        const uint32_t tiles = blocks_for(static_cast<uint32_t>(count), kTileRows);
+        nrc_infer_kernel<<<std::min<uint32_t>(tiles, 2048u), kBlock, 0, st>>>(
+            n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, d_rec, nullptr,
+            static_cast<uint32_t>(count), 0, d_out, nullptr, d_in);
+        e = hipGetLastError();
+    }
+    if (!e) e = hipMemcpyAsync(rgb, d_out, count * 12, hipMemcpyDeviceToHost, st);
+    if (!e && inputs) e = hipMemcpyAsync(inputs, d_in, count * kIn * 4, hipMemcpyDeviceToHost, st);
+    if (!e) e = hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(st);
+    if (d_rec) (void)hipFree(d_rec);
+    if (d_out) (void)hipFree(d_out);
+    if (d_in) (void)hipFree(d_in);
+    return from_hip(e);
+}
+
+// Gradients of the loss over host records (no update), or one Adam step.
+static orn_status host_batch(orn_nrc *n, const orn_nrc_record *records, size_t count, int update, float *loss,
+                             float *mlp_grad, float *grid_grad, size_t grid_len)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if (!records || count == 0 || count > n->train_cap) return ORN_ERROR_INVALID_VALUE;
+    if (grid_grad && grid_len < n->grid_params) return ORN_ERROR_SIZE_MISMATCH;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
+    const uint32_t c = static_cast<uint32_t>(count);
+    ORN_TRY(hipMemcpyAsync(n->train_recs, records, count * sizeof(NrcRecord), hipMemcpyHostToDevice, st));
+    nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
+    ORN_TRY(gradients(n, n->train_recs, nullptr, c, 0u, c, st));
+    if (mlp_grad) ORN_TRY(hipMemcpyAsync(mlp_grad, n->mlp_grad, kMlpParams * 4, hipMemcpyDeviceToHost, st));
+    if (grid_grad)
+        ORN_TRY(hipMemcpyAsync(grid_grad, n->grid_grad, size_t(n->grid_params) * 4, hipMemcpyDeviceToHost, st));
+    if (update) {
+        n->step++;
+        ORN_TRY(apply_params(n, 1, st));
+    } else {
+        const uint64_t gp = n->grid_params;
+        nrc_zero_kernel<<<static_cast<uint32_t>((gp + 255) / 256), 256, 0, st>>>(n->grid_grad, gp);
+    }
+    float l = 0.0f;
+    ORN_TRY(hipMemcpyAsync(&l, n->loss, 4, hipMemcpyDeviceToHost, st));
+    ORN_TRY(hipStreamSynchronize(st));
+    if (loss) *loss = l;
+    return ORN_OK;
+}
+
+orn_status orn_nrc_gradients(orn_nrc *n, const orn_nrc_record *records, size_t count, float *loss, float *mlp_grad,
+                             float *grid_grad, size_t grid_len)
+{
+    return host_batch(n, records, count, 0, loss, mlp_grad, grid_grad, grid_len);
+}
+
+orn_status orn_nrc_step(orn_nrc *n, const orn_nrc_record *records, size_t count, float *loss)
+{
+    return host_batch(n, records, count, 1, loss, nullptr, nullptr, 0);
+}
+
+orn_status orn_nrc_render(orn_nrc *n, orn_target *t, const orn_constant_params *params, uint32_t iterations)
+{
+    if (!n || !t) return ORN_ERROR_INVALID_HANDLE;
+    if (!params) return ORN_ERROR_INVALID_VALUE;
+    if (t->scene != n->scene) return ORN_ERROR_INVALID_HANDLE;
+    if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
+    orn_device_scene *s = n->scene;
+    ORN_TRY(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const uint32_t px = t->width * t->height;
+    if (px > n->query_cap) {  // query and per-pixel sample buffers grow with the target
+        ORN_TRY(hipStreamSynchronize(st));
+        if (n->queries) (void)hipFree(n->queries);
+        if (n->partial) (void)hipFree(n->partial);
+        n->queries = nullptr;
+        n->partial = nullptr;
+        n->query_cap = 0;
+        ORN_TRY(hipMalloc(&n->queries, size_t(px) * sizeof(NrcRecord)));
+        ORN_TRY(hipMalloc(&n->partial, size_t(px) * sizeof(float4)));
+        n->query_cap = px;
+    }
+    DevScene ds = s->ds;
+    ds.stack_slots = s->stack_slots;
+    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+    ORN_TRY(hipEventRecord(n->ev[0], st));
+    float cur = params->current_iteration;
+    const uint32_t iters = std::max(iterations, 1u);
+    for (uint32_t it = 0; it < iters; it++) {
+        nrc_reset_count_kernel<<<1, 64, 0, st>>>(n->counters);
+        ORN_NRC_DISPATCH(launch_paths, s->features, n, ds, *params, t->rng, st, lds);
+        const uint32_t tiles = blocks_for(px, kTileRows);
+        nrc_infer_kernel<<<std::min<uint32_t>(tiles, uint32_t(s->ctx->compute_units) * 2u), kBlock, 0, st>>>(
+            n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, n->queries, n->counters, px, 1,
+            nullptr, n->partial, nullptr);
+        cur = cur + 1.0f;
+        nrc_accumulate_kernel<<<blocks_for(px, 256), 256, 0, st>>>(n->partial, t->acc, px, cur);
+    }
+    nrc_post_kernel<<<blocks_for(px, 256), 256, 0, st>>>(t->acc, t->fb, t->width, t->height, cur,
+                                                         params->inverted_gamma, params->flip_y);
+    ORN_TRY(hipGetLastError());
+    ORN_TRY(hipEventRecord(n->ev[1], st));
+    ORN_TRY(hipStreamSynchronize(st));
+    n->rendered = true;
+    return ORN_OK;
+}
+
+orn_status orn_nrc_train(orn_nrc *n, const orn_constant_params *params, uint32_t frame, uint32_t steps, float *loss)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if (!params || params->width == 0 || params->height == 0) return ORN_ERROR_INVALID_VALUE;
+    orn_device_scene *s = n->scene;
+    ORN_TRY(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    DevScene ds = s->ds;
+    ds.stack_slots = s->stack_slots;
+    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+    ORN_TRY(hipEventRecord(n->ev[2], st));
+    nrc_reset_count_kernel<<<1, 64, 0, st>>>(n->counters + 1);
+    nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
+    if (n->cfg.train_paths) ORN_NRC_DISPATCH(launch_train_paths, s->features, n, ds, *params, frame, st, lds);
+    const uint32_t b = n->cfg.train_batch;
+    for (uint32_t k = 0; k < steps; k++) {
+        // step k takes records k*b, k*b + 1, ... of this frame's samples,
+        // modulo their count (an empty frame makes a step with zero gradient)
+        ORN_TRY(gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
+        n->step++;
+        ORN_TRY(apply_params(n, 1, st));
+    }
+    ORN_TRY(hipEventRecord(n->ev[3], st));
+    n->trained = true;
+    if (loss) {
+        float l = 0.0f;
+        ORN_TRY(hipMemcpyAsync(&l, n->loss, 4, hipMemcpyDeviceToHost, st));
+        ORN_TRY(hipStreamSynchronize(st));
+        *loss = steps ? l / static_cast<float>(steps) : 0.0f;
+    }
+    return ORN_OK;
+}
+
+orn_status orn_nrc_last_ms(orn_nrc *n, float *render_ms, float *train_ms)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    ORN_TRY(hipStreamSynchronize(n->scene->ctx->stream));
+    if (render_ms) *render_ms = 0.0f;
+    if (train_ms) *train_ms = 0.0f;
+    if (render_ms && n->rendered) ORN_TRY(hipEventElapsedTime(render_ms, n->ev[0], n->ev[1]));
+    if (train_ms && n->trained) ORN_TRY(hipEventElapsedTime(train_ms, n->ev[2], n->ev[3]));
+    return ORN_OK;
+}
+
+}  // extern "C"

@@ -28,9 +28,11 @@
 #include <vector>
 
 #include "../../include/ornament_hip.h"
+#include "orn_handles.h"
 #include "orn_trace.h"
 
 using namespace orn;
+using namespace orn_rt;
 
 namespace {
 
@@ -526,78 +528,7 @@ __global__ void zero_kernel(float4 *a, float4 *b, uint32_t n)
 // Host runtime
 // ===========================================================================
 
-// Handles are reference counted: a scene holds its context and a target its
-// scene, so the caller may destroy them in any order (a garbage-collected
-// host language does).
-struct orn_ctx {
-    int device;
-    hipStream_t stream;
-    char name[256];
-    char arch[64];  // gcnArchName, the autotuner's cache key
-    int compute_units;
-    int refs;
-};
-
-struct orn_device_scene {
-    orn_ctx *ctx;
-    DevScene ds;
-    std::vector<void *> allocations;
-    uint32_t stack_slots;   // BVH2 traversal
-    uint32_t stack_slots4;  // BVH4 traversal
-    uint32_t trace_width;   // node width orn_trace_rays uses
-    uint32_t features;      // SceneFeatures present in the arrays
-    uint32_t blas_len;      // BVH2 BLAS nodes
-    int refs;
-};
-
-struct orn_target {
-    orn_device_scene *scene;
-    orn_tuning tuning;
-    uint32_t width, height;
-    float4 *fb;
-    float4 *acc;
-    uint32_t *rng;
-    uint32_t *stage;   // per pixel: launch tag << 16 | jobs finished in that launch
-    uint32_t epoch;    // tag of the latest launch
-    uint32_t *queues;  // one job-queue counter per XCD band
-    uint32_t grid;     // persistent workgroups of the render kernel
-    hipEvent_t ev_start, ev_stop;
-    bool timed;
-};
-
 namespace {
-
-void release_ctx(orn_ctx *c)
-{
-    if (--c->refs > 0) return;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-}
-
-void release_scene(orn_device_scene *s)
-{
-    if (--s->refs > 0) return;
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
-    for (void *p : s->allocations) (void)hipFree(p);
-    orn_ctx *c = s->ctx;
-    delete s;
-    release_ctx(c);
-}
-
-}  // namespace
-
-namespace {
-
-orn_status from_hip(hipError_t e) { return e == hipSuccess ? ORN_OK : static_cast<orn_status>(e); }
-
-#define ORN_TRY(expr)                          \
-    do {                                       \
-        hipError_t e_ = (expr);                \
-        if (e_ != hipSuccess) return from_hip(e_); \
-    } while (0)
 
 orn_status set_device(orn_ctx *ctx) { return from_hip(hipSetDevice(ctx->device)); }
 
@@ -798,7 +729,6 @@ orn_status validate(const orn_scene_desc *d, uint32_t *stack_slots)
     return ORN_OK;
 }
 
-uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 orn_tuning default_tuning()
 {
@@ -865,8 +795,10 @@ hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_
         int blocks = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F, W, BATCH>, kBlock, lds) !=
                 hipSuccess ||
-            blocks < 1)
+            blocks < 1) {
+            (void)hipGetLastError();  // the launch below reports its own errors
             blocks = 1;
+        }
         per_cu = static_cast<uint32_t>(blocks);
     }
     uint32_t grid = cus * per_cu;
@@ -1003,6 +935,19 @@ orn_status upload_scene(orn_ctx *ctx, const orn_scene_desc *desc, uint32_t slots
     s->stack_slots = slots;
     s->blas_len = static_cast<uint32_t>(desc->blas_nodes_len);
     s->trace_width = 2u;
+    for (int a = 0; a < 3; a++) {  // box of all scenes' TLAS roots
+        s->bounds_lo[a] = INFINITY;
+        s->bounds_hi[a] = -INFINITY;
+    }
+    for (size_t r = 0; r < n_roots; r++) {
+        const orn_bvh_node &nd = desc->tlas_nodes[roots[r]];
+        for (int a = 0; a < 3; a++) {
+            const float lo = nd.node_type == 0u ? std::fmin(nd.lmin_or_v0[a], nd.rmin_or_v2[a]) : nd.lmin_or_v0[a];
+            const float hi = nd.node_type == 0u ? std::fmax(nd.lmax_or_v1[a], nd.rmax_or_v3[a]) : nd.lmax_or_v1[a];
+            s->bounds_lo[a] = std::fmin(s->bounds_lo[a], lo);
+            s->bounds_hi[a] = std::fmax(s->bounds_hi[a], hi);
+        }
+    }
     s->features = desc->textures_len ? kHasTextures : 0u;
     for (size_t i = 0; i < desc->tlas_nodes_len; i++) {
         if (desc->tlas_nodes[i].node_type == 1u) s->features |= kHasSpheres;

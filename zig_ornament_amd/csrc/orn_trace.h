@@ -468,6 +468,76 @@ __device__ __forceinline__ f3 refract_dir(f3 e1, f3 e2, float eta)
     return eta * e1 - (eta * d + sqrtf(k)) * e2;
 }
 
+// The shading point of a closest hit (pathtracer.wgsl:114-135,
+// hitrecord.wgsl:10-18): position, front-facing shading normal, material and
+// texture coordinates (uv is only computed when a texture will be read).
+struct Surface {
+    f3 p;
+    f3 normal;
+    bool front;
+    bool textured;
+    float tu, tv;
+    Material m;
+};
+
+// The same operations as the first half of shade() (which keeps its own
+// copy: sharing this function changed the hot kernel's register allocation).
+template <uint32_t F>
+__device__ __forceinline__ Surface surface(const DevScene &sc, uint32_t textures_count, uint32_t tex_base,
+                                           const Ray &ray, const Hit &h)
+{
+    Surface s;
+    s.p = ray.o + h.t * ray.d;
+    s.m = load_material(sc, h.material);
+    const Material &m = s.m;
+    s.textured =
+        (F & kHasTextures) && m.texture - tex_base < textures_count && m.texture < sc.textures_len;
+    s.tu = 0.0f;
+    s.tv = 0.0f;
+    f3 outward;
+    if ((F & kHasSpheres) && (!(F & kHasMeshes) || h.node_type == 1u)) {
+        // sphere: outward = normalize(p - M * (0,0,0,1))
+        const float4 c3 = sc.transforms[4u * (h.inv_tid + 1u) + 3u];
+        const float4 c0 = sc.transforms[4u * (h.inv_tid + 1u)];
+        const float4 c1 = sc.transforms[4u * (h.inv_tid + 1u) + 1u];
+        const float4 c2 = sc.transforms[4u * (h.inv_tid + 1u) + 2u];
+        const f3 center = f3{c0.x * 0.0f + c1.x * 0.0f + c2.x * 0.0f + c3.x * 1.0f,
+                             c0.y * 0.0f + c1.y * 0.0f + c2.y * 0.0f + c3.y * 1.0f,
+                             c0.z * 0.0f + c1.z * 0.0f + c2.z * 0.0f + c3.z * 1.0f};
+        outward = normalize(s.p - center);
+        if (s.textured) {  // uv only reaches the result through a texture fetch
+            const float theta = acos_f(-outward.y);
+            const float phi = atan2_f(-outward.z, outward.x) + kPi;
+            s.tu = phi / (2.0f * kPi);
+            s.tv = theta / kPi;
+        }
+    } else {
+        // mesh: barycentric shading normal and uv, normal by transpose(inverse)
+        const uint32_t i0 = sc.normal_indices[h.tri_id], i1 = sc.normal_indices[h.tri_id + 1u],
+                       i2 = sc.normal_indices[h.tri_id + 2u];
+        const f3 n0 = xyz(sc.normals[i0]), n1 = xyz(sc.normals[i1]), n2 = xyz(sc.normals[i2]);
+        const float w = 1.0f - h.u - h.v;
+        const f3 nrm = (w * n0 + h.u * n1) + h.v * n2;
+        if (s.textured) {
+            const float2 uv0 = sc.uvs[sc.uv_indices[h.tri_id]];
+            const float2 uv1 = sc.uvs[sc.uv_indices[h.tri_id + 1u]];
+            const float2 uv2 = sc.uvs[sc.uv_indices[h.tri_id + 2u]];
+            s.tu = w * uv0.x + h.u * uv1.x + h.v * uv2.x;
+            s.tv = w * uv0.y + h.u * uv1.y + h.v * uv2.y;
+        }
+        outward = normalize(xform_normal(sc.transforms + 4u * h.inv_tid, nrm));
+    }
+    // hitrecord.wgsl:10-18
+    if (dot(ray.d, outward) > 0.0f) {
+        s.normal = -outward;
+        s.front = false;
+    } else {
+        s.normal = outward;
+        s.front = true;
+    }
+    return s;
+}
+
 // Shading of one closest hit: fills the next ray and the attenuation, or the
 // emitted radiance when the path ends (pathtracer.wgsl:114-157).
 // Returns true when the path continues.
@@ -530,6 +600,70 @@ __device__ __forceinline__ bool shade(const DevScene &sc, uint32_t textures_coun
         normal = outward;
         front = true;
     }
+    if (m.type <= 1u) {
+        // Lambertian (material.wgsl:9-20) and metal (:22-27) draw the same
+        // three numbers for random_in_unit_sphere; one code path serves both
+        // kinds of lanes in a wave.
+        const f3 s = random_in_unit_sphere(rng);
+        f3 dir;
+        bool cont;
+        if (m.type == 0u) {
+            dir = normal + normalize(s);
+            if (near_zero(dir)) dir = normal;
+            cont = true;
+        } else {
+            dir = reflect_dir(normalize(ray.d), normal) + m.fuzz * s;
+            cont = dot(dir, normal) > 0.0f;  // absorbed below the surface
+        }
+        next.o = p;
+        next.d = dir;
+        if (cont) {
+            weight = textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo;
+            return true;
+        }
+        weight = f3{0.0f, 0.0f, 0.0f};  // material_emit of a metal
+        return false;
+    }
+    switch (m.type) {
+    case 2u: {  // material.wgsl:36-59
+        const float ratio = front ? (1.0f / m.ior) : m.ior;
+        const f3 ud = normalize(ray.d);
+        const float cos_theta = fminf(dot(-ud, normal), 1.0f);
+        const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+        const bool cannot_refract = ratio * sin_theta > 1.0f;
+        bool reflect = cannot_refract;
+        if (!reflect) {  // the draw happens only when refraction is possible
+            float r0 = (1.0f - ratio) / (1.0f + ratio);
+            r0 = r0 * r0;
+            const float refl = r0 + (1.0f - r0) * pow_f(1.0f - cos_theta, 5.0f);
+            reflect = refl > rng_f32(rng);
+        }
+        next.o = p;
+        next.d = reflect ? reflect_dir(ud, normal) : refract_dir(ud, normal, ratio);
+        weight = f3{1.0f, 1.0f, 1.0f};
+        return true;
+    }
+    default:  // diffuse light (3) emits; anything else emits black
+        weight = (m.type == 3u) ? (textured ? sample_texture(sc, m.texture, tu, tv) : m.albedo)
+                                : f3{0.0f, 0.0f, 0.0f};
+        return false;
+    }
+}
+
+
+// The material part of shade() (material.wgsl:9-89) at a shading point from
+// surface(): the same operations, for callers that need the shading point's
+// data before scattering.
+template <uint32_t F>
+__device__ __forceinline__ bool scatter(const DevScene &sc, const Surface &srf, const Ray &ray, uint32_t &rng,
+                                        Ray &next, f3 &weight)
+{
+    const Material &m = srf.m;
+    const f3 &p = srf.p;
+    const f3 &normal = srf.normal;
+    const bool front = srf.front;
+    const bool textured = srf.textured;
+    const float tu = srf.tu, tv = srf.tv;
     if (m.type <= 1u) {
         // Lambertian (material.wgsl:9-20) and metal (:22-27) draw the same
         // three numbers for random_in_unit_sphere; one code path serves both
