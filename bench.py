@@ -23,6 +23,16 @@ Also reported:
   cpu_baseline the oracle (a C restatement of the WGSL megakernel, OpenMP
                over pixels, serial samples per pixel) timed on the host on a
                bounded sample of the same workload.
+  equal_time_rmse
+               plain path tracing vs the neural radiance cache on the
+               instanced-icosphere scene (1,024 instances of a 327,680-triangle
+               icosphere, two 1024^2 textures, 1920x1080, depth 8): the cache
+               trains online along a seeded 120-frame camera path at 1 spp per
+               frame; the last frame's 1-spp image is compared (RMSE on linear
+               radiance) with a 4,096-spp plain render of that view, next to
+               plain renders of the same view at increasing spp: the plain
+               RMSE in the cache's frame time, and the plain time needed to
+               reach the cache's RMSE.
 """
 import argparse
 import json
@@ -94,6 +104,95 @@ def cpu_baseline(spec, arrays, params, seconds=12.0):
                 stats=stats, bytes_per_sample=algorithmic_bytes(stats) / stats["samples"])
 
 
+def rmse(img, ref):
+    """RMSE over RGB of linear radiance; pixels with a non-finite sample in
+    either image (the reference estimator's rare NaN at the total-internal-
+    reflection boundary) are left out of both."""
+    d = np.asarray(img, np.float64)[:, :3] - np.asarray(ref, np.float64)[:, :3]
+    ok = np.all(np.isfinite(d), axis=1)
+    return float(np.sqrt(np.mean(d[ok] ** 2)))
+
+
+def orbit_lookfrom(spec, frame, frames):
+    """Camera position of a frame of the seeded path: a quarter orbit around
+    the vertical axis through the scene centre, from the scene's own
+    camera position."""
+    lookfrom = np.array(spec.camera[0], np.float64)
+    a = 0.5 * np.pi * frame / max(frames - 1, 1)
+    c, s = np.cos(a), np.sin(a)
+    x, y, z = lookfrom
+    return (float(c * x + s * z), float(y), float(-s * x + c * z))
+
+
+def equal_time_rmse(dev, frames=120, ref_spp=4096):
+    """The cache against plain path tracing at equal GPU time (see the module
+    docstring)."""
+    from zig_ornament_amd.nrc import Nrc
+    spec = W.instanced_icospheres(spp=1)  # 1920x1080, depth 8
+    scene = W.build_scene(spec)
+    ds = zo.DeviceScene(dev, zo.Bvh(scene))
+    lookat, vup = spec.camera[1], spec.camera[2]
+
+    def params_at(frame):
+        scene.set_look_at(orbit_lookfrom(spec, frame, frames), lookat, vup)
+        p = zo._native.ConstantParams()
+        p.camera = scene.camera()
+        p.depth, p.width, p.height = spec.depth, spec.width, spec.height
+        p.flip_y, p.inverted_gamma = 0, 1.0
+        p.ray_cast_epsilon = spec.ray_cast_epsilon
+        p.textures_count = len(spec.textures)
+        p.current_iteration = 0.0
+        return p
+
+    nrc = Nrc(ds, terminate_bounce=0, train_paths=16384, train_batch=16384)
+    n = spec.width * spec.height
+    frame_ms = []
+    t = zo.Target(ds, spec.width, spec.height, 7)
+    for f in range(frames):  # each frame: a new view; 1-spp render beside training paths, one step
+        p = params_at(f)
+        ms, _ = nrc.frame(t, p, 1, f, steps=1)
+        frame_ms.append(ms)
+    img = t.accumulation()
+    p = params_at(frames - 1)
+    ref = zo.Target(ds, spec.width, spec.height, 99)
+    _, _ = ref.autotune(p, 32, cache_path=zo.TUNING_CACHE)
+    done = 0
+    while done < ref_spp:  # progressive renders of at most 512 spp each
+        k = min(512, ref_spp - done)
+        ref.render(_with_iteration(p, done), k)
+        done += k
+    acc = ref.accumulation()
+    reference = acc / acc[:, 3:4]
+    e_nrc = rmse(img, reference)
+    t_nrc = float(np.median(frame_ms[frames // 2:]))
+    plain = []
+    pt = zo.Target(ds, spec.width, spec.height, 7)
+    pt.set_tuning(**ref.tuning())
+    for spp in (1, 2, 3, 4, 6, 8, 12, 16, 24, 32):
+        pt.set_state(np.zeros((n, 4), np.float32), np.arange(n, dtype=np.uint32) + np.uint32(7 * n))
+        pt.render(p, spp)
+        plain.append({"spp": spp, "ms": pt.last_render_ms(), "rmse": rmse(pt.accumulation() / spp, reference)})
+        if plain[-1]["rmse"] <= e_nrc and plain[-1]["ms"] > t_nrc:
+            break
+    within = [q for q in plain if q["ms"] <= t_nrc]
+    match = next((q for q in plain if q["rmse"] <= e_nrc), None)
+    out = {"scene": "instanced_icospheres_1920x1080_depth8", "frames": frames, "reference_spp": ref_spp,
+           "nrc": {"rmse": e_nrc, "ms_per_frame": t_nrc, "spp": 1, "query_bounce": 0,
+                   "train_paths_per_frame": 16384, "adam_steps_per_frame": 1, "batch": 16384},
+           "plain_at_equal_time": within[-1] if within else None,
+           "plain_to_match_nrc_rmse": match, "plain_curve": plain}
+    if match:
+        out["plain_time_over_nrc_time"] = match["ms"] / t_nrc
+    nrc.destroy()
+    return out
+
+
+def _with_iteration(p, it):
+    q = zo._native.ConstantParams.from_buffer_copy(bytes(p))
+    q.current_iteration = float(it)
+    return q
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +206,7 @@ def main():
     ap.add_argument("--tuning-cache", default=zo.TUNING_CACHE,
                     help="autotuner cache (JSON); a scene class missing from it is measured")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch tuning")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the equal-time RMSE leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path tracer shards by pixel tiles with no "
@@ -157,7 +257,8 @@ def main():
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
 
     out = {
-        "metric": "samples/s (plain path tracing, random-spheres 1920x1080 64spp depth 16)",
+        "metric": "samples/s (plain path tracing, random-spheres 1920x1080 64spp depth 16); "
+                  "equal-time RMSE vs 4096-spp reference, plain PT vs NRC",
         "value": value,
         "unit": "samples/s",
         "n_gpus": 1,
@@ -195,6 +296,8 @@ def main():
                                          f"at 64 spp ({cb['pixels'] * spec.spp} samples, "
                                          f"{cb['seconds']:.1f} s) on {cb['threads']} host threads"}
         out["gpu_over_cpu"] = value / cb["samples_per_s"]
+    if not args.no_nrc:
+        out["equal_time_rmse"] = equal_time_rmse(dev)
     print(json.dumps(out), flush=True)
 
 

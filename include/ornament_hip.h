@@ -400,6 +400,13 @@ orn_status orn_nrc_render(orn_nrc *nrc, orn_target *target, const orn_constant_p
  * mean loss of the steps. */
 orn_status orn_nrc_train(orn_nrc *nrc, const orn_constant_params *params, uint32_t frame,
                          uint32_t steps, float *loss);
+/* One frame: the training paths of `frame` on a second stream while the
+ * render (as orn_nrc_render) runs, then `steps` Adam steps; the render uses
+ * the cache as it was before this frame's steps. */
+orn_status orn_nrc_frame(orn_nrc *nrc, orn_target *target, const orn_constant_params *params,
+                         uint32_t iterations, uint32_t frame, uint32_t steps, float *loss);
+/* GPU milliseconds of the latest orn_nrc_frame, from its start to its end. */
+orn_status orn_nrc_frame_ms(orn_nrc *nrc, float *ms);
 /* GPU milliseconds of the latest orn_nrc_render and orn_nrc_train. */
 orn_status orn_nrc_last_ms(orn_nrc *nrc, float *render_ms, float *train_ms);
 /* Parameters (FP32 masters; the kernels read BF16 / FP16 copies made from

@@ -185,3 +185,26 @@ def test_trained_cache_lowers_the_error_of_a_one_spp_image(device):
     e_nrc, e_plain = rmse(a.accumulation(), reference), rmse(b.accumulation(), reference)
     print(f"[nrc] 1-spp RMSE: cache {e_nrc:.4f}, plain {e_plain:.4f}", flush=True)
     assert e_nrc < 0.85 * e_plain  # measured 0.77 on this scene, 0.73 on the 1080p C3 scene
+
+
+def test_frame_renders_with_the_cache_before_its_steps(device):
+    """orn_nrc_frame traces its training paths beside the render: the image
+    is the one orn_nrc_render makes with the same cache, and the steps that
+    follow train like orn_nrc_train (up to the order of atomic sums)."""
+    spec, ds = scene(device, seed=35)
+    p = zo_params(spec)
+    a = Nrc(ds, log2_table=14, terminate_bounce=0, train_paths=4096, train_batch=4096)
+    b = Nrc(ds, log2_table=14, terminate_bounce=0, train_paths=4096, train_batch=4096)
+    ta = zo.Target(ds, spec.width, spec.height, 3)
+    tb = zo.Target(ds, spec.width, spec.height, 3)
+    ms, _ = a.frame(ta, p, 1, 0, steps=1)
+    b.render(tb, p, 1)
+    b.train(p, 0, steps=1)
+    assert ms > 0
+    assert same_bits(ta.accumulation(), tb.accumulation())
+    ma, ga = a.get_params()
+    mb, gb = b.get_params()
+    # Adam's first step moves each parameter by about lr * sign(gradient): only
+    # gradients within rounding of zero may come out differently
+    assert np.mean(np.abs(ma - mb) > 1e-5) < 1e-2
+    assert np.mean(np.abs(ga - gb) > 1e-5) < 1e-2

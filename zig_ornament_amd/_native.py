@@ -155,6 +155,8 @@ DEVICE_API = [
     ("orn_nrc_render", st, [vp, vp, P(ConstantParams), u32]),
     ("orn_nrc_train", st, [vp, P(ConstantParams), u32, u32, fp]),
     ("orn_nrc_last_ms", st, [vp, fp, fp]),
+    ("orn_nrc_frame", st, [vp, vp, P(ConstantParams), u32, u32, u32, fp]),
+    ("orn_nrc_frame_ms", st, [vp, fp]),
     ("orn_nrc_get_params", st, [vp, vp, sz, vp, sz]),
     ("orn_nrc_set_params", st, [vp, vp, sz, vp, sz]),
     ("orn_nrc_query", st, [vp, vp, sz, vp, vp]),

@@ -622,11 +622,25 @@ __device__ __forceinline__ f3 continue_path(const DevScene &sc, const orn_consta
     return thr;  // depth exhausted: the throughput
 }
 
+// Slot of this lane's record among the lanes of its wave that append one:
+// one atomic per wave on the shared counter.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count)
+{
+    const uint64_t m = __ballot(want);
+    if (m == 0u) return 0u;
+    const uint32_t lane = threadIdx.x % 64u;
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    uint32_t base = 0;
+    if (lane == static_cast<uint32_t>(leader)) base = atomicAdd(count, static_cast<uint32_t>(__popcll(m)));
+    base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), leader));
+    return base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
 // One sample per pixel: the reference path up to the hit at bounce k, where a
 // scattering surface becomes a cache query (the sample's value is then
 // throughput x prediction, added by the inference kernel); paths that end
 // earlier (sky, emitter, absorption, depth) keep their exact value.
-template <uint32_t F>
+template <uint32_t F, bool PRIMARY>
 __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
                                                            uint32_t *__restrict__ rng_buf,
                                                            float4 *__restrict__ partial,
@@ -635,47 +649,79 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
 {
     extern __shared__ uint32_t lds_stack[];
     uint32_t *stack = lds_stack + threadIdx.x;
-    const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= p.width * p.height) return;
-    const uint32_t x = pix % p.width, y = pix / p.width;
-    uint32_t rng = rng_buf[pix];
-    Ray ray = camera_ray(p, x, y, rng);
-    f3 thr = mk(1.0f, 1.0f, 1.0f);
-    f3 color = thr;
-    bool done = false;
-    for (uint32_t b = 0; b < p.depth && !done; b++) {
-        Hit h{};
-        if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
-            color = thr * sky(ray.d);
+    // a workgroup covers a 16 x 16 tile, a wave a 16 x 4 strip (coherent rays)
+    const uint32_t tiles_x = (p.width + 15u) / 16u;
+    const uint32_t x = (blockIdx.x % tiles_x) * 16u + threadIdx.x % 16u;
+    const uint32_t y = (blockIdx.x / tiles_x) * 16u + threadIdx.x / 16u;
+    const bool active = x < p.width && y < p.height;
+    const uint32_t pix = y * p.width + x;
+    bool query = false;
+    NrcRecord rc;
+    if (active) {
+        uint32_t rng = rng_buf[pix];
+        Ray ray = camera_ray(p, x, y, rng);
+        f3 thr = mk(1.0f, 1.0f, 1.0f);
+        f3 color = thr;
+        bool done = false;
+        if constexpr (PRIMARY) {  // query at the primary hit: no scattering at all
+            Hit h{};
             done = true;
-            break;
+            if (p.depth == 0u) {
+                done = false;
+            } else if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
+                color = thr * sky(ray.d);
+            } else {
+                const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
+                if (s.m.type <= 2u) {
+                    fill_record<F>(sc, s, ray, rc);
+                    rc.pixel = pix;
+                    rc.weight[0] = 1.0f;
+                    rc.weight[1] = 1.0f;
+                    rc.weight[2] = 1.0f;
+                    query = true;
+                    color = mk(0.0f, 0.0f, 0.0f);
+                } else {  // material_emit (material.wgsl:76-89)
+                    color = s.m.type == 3u ? (s.textured ? sample_texture(sc, s.m.texture, s.tu, s.tv) : s.m.albedo)
+                                           : mk(0.0f, 0.0f, 0.0f);
+                    color = thr * color;
+                }
+            }
         }
-        const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
-        if (b == k && s.m.type <= 2u) {
-            NrcRecord rc;
-            fill_record<F>(sc, s, ray, rc);
-            rc.pixel = pix;
-            rc.weight[0] = thr.x;
-            rc.weight[1] = thr.y;
-            rc.weight[2] = thr.z;
-            queries[atomicAdd(query_count, 1u)] = rc;
-            color = mk(0.0f, 0.0f, 0.0f);
-            done = true;
-            break;
+        for (uint32_t b = 0; !PRIMARY && b < p.depth && !done; b++) {
+            Hit h{};
+            if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
+                color = thr * sky(ray.d);
+                done = true;
+                break;
+            }
+            const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
+            if (b == k && s.m.type <= 2u) {
+                fill_record<F>(sc, s, ray, rc);
+                rc.pixel = pix;
+                rc.weight[0] = thr.x;
+                rc.weight[1] = thr.y;
+                rc.weight[2] = thr.z;
+                query = true;
+                color = mk(0.0f, 0.0f, 0.0f);
+                done = true;
+                break;
+            }
+            Ray next;
+            f3 w;
+            if (scatter<F>(sc, s, ray, rng, next, w)) {
+                ray = next;
+                thr = thr * w;
+            } else {
+                color = thr * w;
+                done = true;
+            }
         }
-        Ray next;
-        f3 w;
-        if (scatter<F>(sc, s, ray, rng, next, w)) {
-            ray = next;
-            thr = thr * w;
-        } else {
-            color = thr * w;
-            done = true;
-        }
+        if (!done) color = thr;
+        partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+        rng_buf[pix] = rng;
     }
-    if (!done) color = thr;
-    partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
-    rng_buf[pix] = rng;
+    const uint32_t slot = wave_append(query, query_count);
+    if (query) queries[slot] = rc;
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x)  // lowbias32 (Wellons)
@@ -701,34 +747,38 @@ __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, or
     extern __shared__ uint32_t lds_stack[];
     uint32_t *stack = lds_stack + threadIdx.x;
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= n_paths) return;
-    const uint32_t npx = p.width * p.height;
-    const uint32_t pix = mix32(j ^ mix32(frame * 0x9e3779b9u + 0x632be5abu)) % npx;
-    uint32_t rng = mix32(pix + mix32(frame ^ 0x85ebca6bu) + j * 0xc2b2ae35u);
-    Ray ray = camera_ray(p, pix % p.width, pix / p.width, rng);
-    for (uint32_t b = 0; b <= k && b < p.depth; b++) {
-        Hit h{};
-        if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) return;
-        const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
-        Ray next;
-        f3 w;
-        if (b == k) {
-            if (s.m.type > 2u) return;  // emitters are exact, never queried
-            NrcRecord rc;
-            fill_record<F>(sc, s, ray, rc);
-            rc.pixel = pix;
-            f3 v;
-            if (scatter<F>(sc, s, ray, rng, next, w)) v = continue_path<F>(sc, p, next, w, b + 1u, rng, stack);
-            else v = w;
-            rc.weight[0] = v.x;
-            rc.weight[1] = v.y;
-            rc.weight[2] = v.z;
-            recs[atomicAdd(rec_count, 1u)] = rc;
-            return;
+    bool keep = false;
+    NrcRecord rc;
+    if (j < n_paths) {
+        const uint32_t npx = p.width * p.height;
+        const uint32_t pix = mix32(j ^ mix32(frame * 0x9e3779b9u + 0x632be5abu)) % npx;
+        uint32_t rng = mix32(pix + mix32(frame ^ 0x85ebca6bu) + j * 0xc2b2ae35u);
+        Ray ray = camera_ray(p, pix % p.width, pix / p.width, rng);
+        for (uint32_t b = 0; b <= k && b < p.depth; b++) {
+            Hit h{};
+            if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) break;
+            const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
+            Ray next;
+            f3 w;
+            if (b == k) {
+                if (s.m.type > 2u) break;  // emitters are exact, never queried
+                fill_record<F>(sc, s, ray, rc);
+                rc.pixel = pix;
+                f3 v;
+                if (scatter<F>(sc, s, ray, rng, next, w)) v = continue_path<F>(sc, p, next, w, b + 1u, rng, stack);
+                else v = w;
+                rc.weight[0] = v.x;
+                rc.weight[1] = v.y;
+                rc.weight[2] = v.z;
+                keep = true;
+                break;
+            }
+            if (!scatter<F>(sc, s, ray, rng, next, w)) break;
+            ray = next;
         }
-        if (!scatter<F>(sc, s, ray, rng, next, w)) return;
-        ray = next;
     }
+    const uint32_t slot = wave_append(keep, rec_count);
+    if (keep) recs[slot] = rc;
 }
 
 // render() bookkeeping of one sample per pixel (pathtracer.wgsl:64-79):
@@ -798,7 +848,9 @@ struct orn_nrc {
     float *loss;         // [0] loss sum of the current step
     uint32_t step;       // Adam steps taken
     hipEvent_t ev[4];    // render start/stop, train start/stop
-    bool rendered, trained;
+    hipEvent_t fork, join, frame_start, frame_stop;
+    hipStream_t side;    // training paths of orn_nrc_frame, beside the render
+    bool rendered, trained, framed;
 };
 
 namespace {
@@ -807,9 +859,12 @@ template <uint32_t F>
 void launch_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, uint32_t *rng, hipStream_t st,
                   size_t lds)
 {
-    const uint32_t px = p.width * p.height;
-    nrc_paths_kernel<F><<<blocks_for(px, kBlock), kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, n->partial,
-                                                                     n->queries, n->counters);
+    const uint32_t tiles = blocks_for(p.width, 16u) * blocks_for(p.height, 16u);
+    if (n->cfg.terminate_bounce == 0u)
+        nrc_paths_kernel<F, true><<<tiles, kBlock, lds, st>>>(ds, p, 0u, rng, n->partial, n->queries, n->counters);
+    else
+        nrc_paths_kernel<F, false><<<tiles, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, n->partial,
+                                                               n->queries, n->counters);
 }
 
 template <uint32_t F>
@@ -843,6 +898,12 @@ void free_nrc(orn_nrc *n)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : n->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {n->fork, n->join, n->frame_start, n->frame_stop})
+        if (e) (void)hipEventDestroy(e);
+    if (n->side) {
+        (void)hipStreamSynchronize(n->side);
+        (void)hipStreamDestroy(n->side);
+    }
     orn_device_scene *s = n->scene;
     delete n;
     release_scene(s);
@@ -977,6 +1038,10 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         return fail(e);
     for (hipEvent_t &ev : n->ev)
         if ((e = hipEventCreate(&ev))) return fail(e);
+    if ((e = hipEventCreateWithFlags(&n->fork, hipEventDisableTiming)) ||
+        (e = hipEventCreateWithFlags(&n->join, hipEventDisableTiming)) || (e = hipEventCreate(&n->frame_start)) ||
+        (e = hipEventCreate(&n->frame_stop)) || (e = hipStreamCreateWithFlags(&n->side, hipStreamNonBlocking)))
+        return fail(e);
     // initial parameters: MLP uniform in +-sqrt(6 / (fan_in + fan_out)),
     // grid uniform in +-1e-4
     SplitMix rng{cfg->seed};
@@ -1126,14 +1191,14 @@ orn_status orn_nrc_step(orn_nrc *n, const orn_nrc_record *records, size_t count,
     return host_batch(n, records, count, 1, loss, nullptr, nullptr, 0);
 }
 
-orn_status orn_nrc_render(orn_nrc *n, orn_target *t, const orn_constant_params *params, uint32_t iterations)
+}  // extern "C"
+
+namespace {
+
+// One render() with cached termination, enqueued on the scene's stream.
+orn_status enqueue_render(orn_nrc *n, orn_target *t, const orn_constant_params *params, uint32_t iterations)
 {
-    if (!n || !t) return ORN_ERROR_INVALID_HANDLE;
-    if (!params) return ORN_ERROR_INVALID_VALUE;
-    if (t->scene != n->scene) return ORN_ERROR_INVALID_HANDLE;
-    if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
     orn_device_scene *s = n->scene;
-    ORN_TRY(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     const uint32_t px = t->width * t->height;
     if (px > n->query_cap) {  // query and per-pixel sample buffers grow with the target
@@ -1150,7 +1215,6 @@ orn_status orn_nrc_render(orn_nrc *n, orn_target *t, const orn_constant_params *
     DevScene ds = s->ds;
     ds.stack_slots = s->stack_slots;
     const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
-    ORN_TRY(hipEventRecord(n->ev[0], st));
     float cur = params->current_iteration;
     const uint32_t iters = std::max(iterations, 1u);
     for (uint32_t it = 0; it < iters; it++) {
@@ -1166,6 +1230,59 @@ orn_status orn_nrc_render(orn_nrc *n, orn_target *t, const orn_constant_params *
     nrc_post_kernel<<<blocks_for(px, 256), 256, 0, st>>>(t->acc, t->fb, t->width, t->height, cur,
                                                          params->inverted_gamma, params->flip_y);
     ORN_TRY(hipGetLastError());
+    return ORN_OK;
+}
+
+// Training paths of a frame, enqueued on stream st.
+orn_status enqueue_train_paths(orn_nrc *n, const orn_constant_params *params, uint32_t frame, hipStream_t st)
+{
+    orn_device_scene *s = n->scene;
+    DevScene ds = s->ds;
+    ds.stack_slots = s->stack_slots;
+    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+    nrc_reset_count_kernel<<<1, 64, 0, st>>>(n->counters + 1);
+    if (n->cfg.train_paths) ORN_NRC_DISPATCH(launch_train_paths, s->features, n, ds, *params, frame, st, lds);
+    ORN_TRY(hipGetLastError());
+    return ORN_OK;
+}
+
+// `steps` Adam steps on the frame's training records, on the scene's stream.
+orn_status enqueue_steps(orn_nrc *n, uint32_t steps)
+{
+    hipStream_t st = n->scene->ctx->stream;
+    nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
+    const uint32_t b = n->cfg.train_batch;
+    for (uint32_t k = 0; k < steps; k++) {
+        // step k takes records k*b, k*b + 1, ... of this frame's samples,
+        // modulo their count (an empty frame makes a step with zero gradient)
+        ORN_TRY(gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
+        n->step++;
+        ORN_TRY(apply_params(n, 1, st));
+    }
+    return ORN_OK;
+}
+
+orn_status check_target(orn_nrc *n, orn_target *t, const orn_constant_params *params)
+{
+    if (!n || !t) return ORN_ERROR_INVALID_HANDLE;
+    if (!params) return ORN_ERROR_INVALID_VALUE;
+    if (t->scene != n->scene) return ORN_ERROR_INVALID_HANDLE;
+    if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
+    return ORN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+orn_status orn_nrc_render(orn_nrc *n, orn_target *t, const orn_constant_params *params, uint32_t iterations)
+{
+    orn_status r = check_target(n, t, params);
+    if (r != ORN_OK) return r;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
+    ORN_TRY(hipEventRecord(n->ev[0], st));
+    if ((r = enqueue_render(n, t, params, iterations)) != ORN_OK) return r;
     ORN_TRY(hipEventRecord(n->ev[1], st));
     ORN_TRY(hipStreamSynchronize(st));
     n->rendered = true;
@@ -1176,24 +1293,12 @@ orn_status orn_nrc_train(orn_nrc *n, const orn_constant_params *params, uint32_t
 {
     if (!n) return ORN_ERROR_INVALID_HANDLE;
     if (!params || params->width == 0 || params->height == 0) return ORN_ERROR_INVALID_VALUE;
-    orn_device_scene *s = n->scene;
-    ORN_TRY(hipSetDevice(s->ctx->device));
-    hipStream_t st = s->ctx->stream;
-    DevScene ds = s->ds;
-    ds.stack_slots = s->stack_slots;
-    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
     ORN_TRY(hipEventRecord(n->ev[2], st));
-    nrc_reset_count_kernel<<<1, 64, 0, st>>>(n->counters + 1);
-    nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
-    if (n->cfg.train_paths) ORN_NRC_DISPATCH(launch_train_paths, s->features, n, ds, *params, frame, st, lds);
-    const uint32_t b = n->cfg.train_batch;
-    for (uint32_t k = 0; k < steps; k++) {
-        // step k takes records k*b, k*b + 1, ... of this frame's samples,
-        // modulo their count (an empty frame makes a step with zero gradient)
-        ORN_TRY(gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
-        n->step++;
-        ORN_TRY(apply_params(n, 1, st));
-    }
+    orn_status r = enqueue_train_paths(n, params, frame, st);
+    if (r == ORN_OK) r = enqueue_steps(n, steps);
+    if (r != ORN_OK) return r;
     ORN_TRY(hipEventRecord(n->ev[3], st));
     n->trained = true;
     if (loss) {
@@ -1202,6 +1307,43 @@ orn_status orn_nrc_train(orn_nrc *n, const orn_constant_params *params, uint32_t
         ORN_TRY(hipStreamSynchronize(st));
         *loss = steps ? l / static_cast<float>(steps) : 0.0f;
     }
+    return ORN_OK;
+}
+
+orn_status orn_nrc_frame(orn_nrc *n, orn_target *t, const orn_constant_params *params, uint32_t iterations,
+                         uint32_t frame, uint32_t steps, float *loss)
+{
+    orn_status r = check_target(n, t, params);
+    if (r != ORN_OK) return r;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    hipStream_t st = n->scene->ctx->stream;
+    ORN_TRY(hipEventRecord(n->frame_start, st));
+    ORN_TRY(hipEventRecord(n->fork, st));
+    ORN_TRY(hipStreamWaitEvent(n->side, n->fork, 0));
+    if ((r = enqueue_train_paths(n, params, frame, n->side)) != ORN_OK) return r;
+    ORN_TRY(hipEventRecord(n->join, n->side));
+    if ((r = enqueue_render(n, t, params, iterations)) != ORN_OK) return r;
+    ORN_TRY(hipStreamWaitEvent(st, n->join, 0));  // the steps read the records; the render read the weights
+    if ((r = enqueue_steps(n, steps)) != ORN_OK) return r;
+    ORN_TRY(hipEventRecord(n->frame_stop, st));
+    n->framed = true;
+    if (loss) {
+        float l = 0.0f;
+        ORN_TRY(hipMemcpyAsync(&l, n->loss, 4, hipMemcpyDeviceToHost, st));
+        ORN_TRY(hipStreamSynchronize(st));
+        *loss = steps ? l / static_cast<float>(steps) : 0.0f;
+    }
+    ORN_TRY(hipStreamSynchronize(st));
+    return ORN_OK;
+}
+
+orn_status orn_nrc_frame_ms(orn_nrc *n, float *ms)
+{
+    if (!n) return ORN_ERROR_INVALID_HANDLE;
+    if (!ms) return ORN_ERROR_INVALID_VALUE;
+    *ms = 0.0f;
+    ORN_TRY(hipSetDevice(n->scene->ctx->device));
+    if (n->framed) ORN_TRY(hipEventElapsedTime(ms, n->frame_start, n->frame_stop));
     return ORN_OK;
 }
 

@@ -379,6 +379,7 @@ __device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const R
     return tr.hit;
 }
 
+
 // ---------------------------------------------------------------------------
 // Shading: material.wgsl, hitrecord.wgsl, pathtracer.wgsl:95-161
 // ---------------------------------------------------------------------------

@@ -94,6 +94,16 @@ class Nrc:
                                   C.byref(loss) if want_loss else None), "orn_nrc_train")
         return loss.value if want_loss else None
 
+    def frame(self, target, params, iterations, frame, steps=1, want_loss=False):
+        """orn_nrc_frame: training paths beside the render, then Adam steps;
+        returns (frame GPU ms, loss or None)."""
+        loss = C.c_float()
+        check(lib().orn_nrc_frame(self._h, target.handle, C.byref(params), int(iterations), int(frame),
+                                  int(steps), C.byref(loss) if want_loss else None), "orn_nrc_frame")
+        ms = C.c_float()
+        check(lib().orn_nrc_frame_ms(self._h, C.byref(ms)), "orn_nrc_frame_ms")
+        return ms.value, (loss.value if want_loss else None)
+
     def last_ms(self):
         r, t = C.c_float(), C.c_float()
         check(lib().orn_nrc_last_ms(self._h, C.byref(r), C.byref(t)), "orn_nrc_last_ms")
