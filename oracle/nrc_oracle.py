@@ -23,12 +23,15 @@ the same rounding points as the gfx950 kernels:
             W(l)^T) * [x(l) > 0]); grid gradient from dx0 = dY0 W0^T (f32)
             scattered to the 8 corners with the trilinear weights
   Adam      m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; p -= lr (m / (1-b1^t)) /
-            (sqrt(v / (1-b2^t)) + eps)
+            (sqrt(v / (1-b2^t)) + eps); hash-grid parameters whose gradient
+            is exactly 0 skip the step (value and moments kept, t still
+            counts every step: Mueller et al. 2022, sec. 4)
 
 Independent checks in tests/test_nrc_oracle.py pin this restatement: the
 hash and dense indices against closed forms, the encoding's linearity and
-partition of unity, and the analytic backward against torch float64
-autograd (straight-through rounding) and finite differences.
+partition of unity, the analytic backward against torch float64 autograd
+(straight-through rounding) and finite differences, Adam against
+torch.optim.Adam and the skipping form against torch.optim.SparseAdam.
 """
 import numpy as np
 
@@ -175,12 +178,16 @@ def grid_gradient(info, pos, dx0):
     return g.reshape(-1)
 
 
-def adam(p, g, m, v, t, lr, b1, b2, eps):
-    """One Adam step in float64; returns (p, m, v)."""
-    m = b1 * m + (1 - b1) * g
-    v = b2 * v + (1 - b2) * g * g
-    p = p - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
-    return p, m, v
+def adam(p, g, m, v, t, lr, b1, b2, eps, skip_zero=False):
+    """One Adam step in float64; returns (p, m, v). With skip_zero, entries
+    whose gradient is exactly 0 keep p, m and v (the hash grid's rule)."""
+    m1 = b1 * m + (1 - b1) * g
+    v1 = b2 * v + (1 - b2) * g * g
+    p1 = p - lr * (m1 / (1 - b1 ** t)) / (np.sqrt(v1 / (1 - b2 ** t)) + eps)
+    if not skip_zero:
+        return p1, m1, v1
+    keep = np.asarray(g) == 0
+    return (np.where(keep, p, p1), np.where(keep, m, m1), np.where(keep, v, v1))
 
 
 def torch_reference(mlp, x_in, target, straight_through=True):

@@ -116,19 +116,37 @@ def test_gradients_match_oracle_and_autograd(cache):
     assert np.all(gg[np.setdiff1d(np.arange(gg.size), touched)] == 0)
 
 
-def test_adam_step_matches_oracle(device):
+def test_adam_steps_match_oracle(device):
+    """Two Adam steps on different batches: the MLP takes dense steps; grid
+    parameters with an exactly-zero gradient in the second batch keep the
+    value (and moments) the first step left, although their moments would
+    move them under dense Adam."""
     spec, ds = scene(device, seed=32)
     nrc = Nrc(ds, log2_table=12, train_batch=512, learning_rate=1e-3)
     mlp0, grid0 = random_params(nrc, 9)
-    rec = records(nrc.info(), 512, 4)
-    _, mg, gg = nrc.gradients(rec)
-    nrc.step(rec)
-    mlp1, grid1 = nrc.get_params()
-    p, _, _ = R.adam(mlp0.astype(np.float64), mg.astype(np.float64), 0.0, 0.0, 1, 1e-3, 0.9, 0.99, 1e-15)
-    assert np.allclose(mlp1, p, rtol=0, atol=2e-6)
-    p, _, _ = R.adam(grid0.astype(np.float64), gg.astype(np.float64), 0.0, 0.0, 1, 1e-3, 0.9, 0.99, 1e-15)
-    assert np.allclose(grid1, p, rtol=0, atol=2e-6)
-    assert nrc.info()["adam_steps"] == 1
+    info = nrc.info()
+    hp = (1e-3, 0.9, 0.99, 1e-15)
+    zm, zg = np.zeros(mlp0.size), np.zeros(grid0.size)
+    state = dict(mlp=(zm, zm), grid=(zg, zg))
+    params = (mlp0, grid0)
+    for t, seed in ((1, 4), (2, 5)):
+        rec = records(info, 512, seed)
+        _, mg, gg = nrc.gradients(rec)
+        nrc.step(rec)
+        mlp1, grid1 = nrc.get_params()
+        p, m, v = R.adam(params[0].astype(np.float64), mg.astype(np.float64), *state["mlp"], t, *hp)
+        assert np.allclose(mlp1, p, rtol=0, atol=2e-6), t
+        state["mlp"] = (m, v)
+        p, m, v = R.adam(params[1].astype(np.float64), gg.astype(np.float64), *state["grid"], t, *hp,
+                         skip_zero=True)
+        assert np.allclose(grid1, p, rtol=0, atol=2e-6), t
+        skipped = gg == 0
+        assert np.array_equal(grid1[skipped], params[1][skipped]), t
+        if t == 2:  # entries the first step moved and the second skipped
+            assert np.count_nonzero(skipped & (state["grid"][0] != 0)) > 100
+        state["grid"] = (m, v)
+        params = (mlp1, grid1)
+    assert nrc.info()["adam_steps"] == 2
 
 
 def test_training_lowers_the_loss(device):

@@ -2,7 +2,8 @@
 encoding by closed forms (dense levels reproduce linear functions exactly,
 weights are a partition of unity, dense indices are a bijection), the
 analytic backward against torch float64 autograd and finite differences,
-and the Adam restatement against torch.optim.Adam."""
+and the Adam restatement against torch.optim.Adam (and its zero-skipping
+form, used for the hash grid, against torch.optim.SparseAdam)."""
 import os
 import sys
 
@@ -201,3 +202,31 @@ def test_adam_matches_torch():
         opt.step()
         p, m, v = R.adam(p, g, m, v, t, 1e-2, 0.9, 0.99, 1e-15)
     assert np.allclose(p, pt.detach().numpy(), rtol=0, atol=1e-12)
+
+
+def test_skipping_adam_matches_sparse_adam():
+    """Entries with an exactly-zero gradient keep value and moments while the
+    step count advances: torch.optim.SparseAdam's update of the entries a
+    sparse gradient holds. (SparseAdam adds eps before the bias correction
+    of the denominator; at eps = 1e-15 against moments ~1e-2 that is far
+    below the tolerance.)"""
+    rng = np.random.default_rng(7)
+    n = 64
+    p0 = rng.normal(size=n)
+    pt = torch.tensor(p0.copy(), requires_grad=True)
+    opt = torch.optim.SparseAdam([pt], lr=1e-2, betas=(0.9, 0.99), eps=1e-15)
+    p, m, v = p0.copy(), np.zeros(n), np.zeros(n)
+    skipped_with_moments = 0
+    for t in range(1, 8):
+        g = rng.normal(size=n)
+        g[rng.random(n) < 0.5] = 0.0
+        nz = np.nonzero(g)[0]
+        skipped_with_moments += int(np.sum((g == 0) & (m != 0)))
+        pt.grad = torch.sparse_coo_tensor(torch.from_numpy(nz[None]), torch.from_numpy(g[nz]), (n,))
+        opt.step()
+        p, m, v = R.adam(p, g, m, v, t, 1e-2, 0.9, 0.99, 1e-15, skip_zero=True)
+        assert np.allclose(p, pt.detach().numpy(), rtol=1e-12, atol=1e-12), t
+    assert skipped_with_moments > 0  # the skip was exercised where dense Adam would move
+    # and the dense form differs there
+    q, _, _ = R.adam(p0, np.where(np.arange(n) % 2, 1.0, 0.0), np.ones(n), np.ones(n), 2, 1e-2, 0.9, 0.99, 1e-15)
+    assert not np.allclose(q[::2], p0[::2])

@@ -360,15 +360,18 @@ __global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __ha
 // batch of `batch` samples, records first, first + 1, ... taken modulo the
 // record count (a frame with fewer records than a step needs reuses them;
 // none: an empty step). Writes the tile's weight gradient to
-// partials[blockIdx.x], adds the grid gradient with atomics and the tile's
-// loss to loss_sum.
+// partials[blockIdx.x], the tile's loss to loss_sum, and for batch sample b
+// its unit position (sample_u[b]) and the loss gradient of its grid features
+// (sample_dx[l * dx_stride + b], both features of level l), from which
+// nrc_grid_kernel forms the grid gradient without global atomics.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __half2 *__restrict__ table,
                                                            const uint16_t *__restrict__ wt,
                                                            const NrcRecord *__restrict__ recs,
                                                            const uint32_t *__restrict__ count_ptr, uint32_t count_max,
                                                            uint32_t first, uint32_t batch,
-                                                           float *__restrict__ partials, float *__restrict__ grid_grad,
+                                                           float *__restrict__ partials, float4 *__restrict__ sample_u,
+                                                           float2 *__restrict__ sample_dx, uint32_t dx_stride,
                                                            float *__restrict__ loss_sum)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtElems];
@@ -448,36 +451,22 @@ __global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __ha
         b ^= 1u;
         __syncthreads();
     }
-    // grid gradient: the 8 corners of every level of every sample
+    // per-sample inputs of the grid gradient: lane group (sample s, level
+    // quarter) writes 4 levels; consecutive samples are consecutive addresses
     {
-        const uint32_t s = threadIdx.x / 4u, part4 = threadIdx.x % 4u;
+        const uint32_t s = threadIdx.x % kTileRows, part4 = threadIdx.x / kTileRows;
         if (s < valid) {
-            const uint32_t i = tile_record(tile_base, s, total);
-            float u[3];
-            unit_position(g, recs[i].pos, u);
-            for (uint32_t l = part4 * 4u; l < part4 * 4u + 4u; l++) {
-                uint32_t idx[8];
-                float w[8];
-                level_corners(g, l, u, idx, w);
-                const float d0 = s_dx0[s * kGridFeatures + 2u * l], d1 = s_dx0[s * kGridFeatures + 2u * l + 1u];
-                float *gg = grid_grad + (size_t(l) << g.log2_table) * 2u;
-                for (int c = 0; c < 8; c++) {
-                    if (d0 != 0.0f) atomicAdd(gg + 2u * idx[c], w[c] * d0);
-                    if (d1 != 0.0f) atomicAdd(gg + 2u * idx[c] + 1u, w[c] * d1);
-                }
+            const uint32_t b = tile_rel + s;
+            if (part4 == 0u) {
+                float u[3];
+                unit_position(g, recs[tile_record(tile_base, s, total)].pos, u);
+                sample_u[b] = make_float4(u[0], u[1], u[2], 0.0f);
             }
+            for (uint32_t l = part4 * 4u; l < part4 * 4u + 4u; l++)
+                sample_dx[size_t(l) * dx_stride + b] =
+                    make_float2(s_dx0[s * kGridFeatures + 2u * l], s_dx0[s * kGridFeatures + 2u * l + 1u]);
         }
     }
-}
-
-// dW = sum over workgroup partials (partials[wg][j])
-__global__ void nrc_reduce_kernel(const float *__restrict__ partials, uint32_t groups, float *__restrict__ grad)
-{
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= kMlpParams) return;
-    float s = 0.0f;
-    for (uint32_t g = 0; g < groups; g++) s = s + partials[size_t(g) * kMlpParams + j];
-    grad[j] = s;
 }
 
 struct AdamStep {
@@ -493,20 +482,45 @@ __device__ __forceinline__ float adam(float p, float g, float &m, float &v, cons
     return p - a.lr * mh / (sqrtf(vh) + a.eps);
 }
 
-// MLP parameters: master (layout [l][k][n], W5 [k][n]) and the transposed,
-// padded bf16 copy the kernels read.
-__global__ void nrc_adam_mlp_kernel(AdamStep a, float *__restrict__ w, const float *__restrict__ grad,
-                                    float *__restrict__ m, float *__restrict__ v, uint16_t *__restrict__ wt, int update)
+// What the gradient kernels do with a batch's gradient.
+enum GradMode : int {
+    kRefresh = 0,   // no gradient: only rewrite the working copies from the masters
+    kGradient = 1,  // store the gradient (orn_nrc_gradients)
+    kUpdate = 2,    // one Adam step
+};
+
+// MLP parameters: dW = sum over the training workgroups' partials
+// (partials[group][j], summed in a fixed order: groups slice, slice + 4, ...
+// per wave, then the four wave sums), then store it or take an Adam step,
+// and rewrite the transposed, padded bf16 copy the kernels read. One
+// workgroup covers 64 parameters; the parameter count is a multiple of 64.
+static_assert(kMlpParams % 64u == 0u, "MLP parameters per workgroup");
+__global__ __launch_bounds__(kBlock) void nrc_mlp_kernel(const float *__restrict__ partials, uint32_t groups,
+                                                         AdamStep a, float *__restrict__ w, float *__restrict__ grad,
+                                                         float *__restrict__ m, float *__restrict__ v,
+                                                         uint16_t *__restrict__ wt, int mode)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= kMlpParams) return;
+    __shared__ float s_sum[kBlock / 64u][64];
+    const uint32_t lane = threadIdx.x % 64u, slice = threadIdx.x / 64u;
+    const uint32_t j = blockIdx.x * 64u + lane;
+    if (mode != kRefresh) {
+        float acc = 0.0f;
+        for (uint32_t gi = slice; gi < groups; gi += kBlock / 64u) acc = acc + partials[size_t(gi) * kMlpParams + j];
+        s_sum[slice][lane] = acc;
+    }
+    __syncthreads();
+    if (slice != 0u) return;
     float p = w[j];
-    if (update) {
-        float mj = m[j], vj = v[j];
-        p = adam(p, grad[j], mj, vj, a);
-        m[j] = mj;
-        v[j] = vj;
-        w[j] = p;
+    if (mode != kRefresh) {
+        const float gj = (s_sum[0][lane] + s_sum[1][lane]) + (s_sum[2][lane] + s_sum[3][lane]);
+        grad[j] = gj;
+        if (mode == kUpdate) {
+            float mj = m[j], vj = v[j];
+            p = adam(p, gj, mj, vj, a);
+            m[j] = mj;
+            v[j] = vj;
+            w[j] = p;
+        }
     }
     uint32_t dst;
     if (j < kOutOffset) {
@@ -519,23 +533,125 @@ __global__ void nrc_adam_mlp_kernel(AdamStep a, float *__restrict__ w, const flo
     wt[dst] = to_bf16(p);
 }
 
-// Grid parameters: master fp32, fp16 copy; the gradient is cleared.
-__global__ void nrc_adam_grid_kernel(AdamStep a, float *__restrict__ w, float *__restrict__ grad,
-                                     float *__restrict__ m, float *__restrict__ v, __half *__restrict__ half_w,
-                                     uint64_t n, int update)
+// fp16 working copy of the grid masters
+__global__ void nrc_grid_half_kernel(const float *__restrict__ w, __half *__restrict__ half_w, uint64_t n)
 {
     const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    float p = w[j];
-    if (update) {
-        float mj = m[j], vj = v[j];
-        p = adam(p, grad[j], mj, vj, a);
-        m[j] = mj;
-        v[j] = vj;
-        w[j] = p;
-        grad[j] = 0.0f;
+    if (j < n) half_w[j] = __float2half_rn(w[j]);
+}
+
+// Grid gradient and update without global atomics. A workgroup owns
+// kPartEntries consecutive table entries of one level and one chunk of the
+// batch, and sums into LDS the contributions w_c * dL/df of the chunk's
+// samples whose cell has a corner in its range (sample b: unit position
+// sample_u[b], feature gradients sample_dx[l * dx_stride + b]). Levels whose
+// table is small next to the others (the coarse, densely indexed ones, where
+// nearly every corner lands in one range) split the batch into chunks so
+// that every workgroup sums about as many contributions; their chunk sums
+// go to scratch and nrc_grid_finish_kernel adds them in chunk order.
+// Finishing a parameter: kGradient stores the sum into grad; kUpdate takes an
+// Adam step, skipping parameters whose gradient is exactly 0 (they keep value
+// and moments, as in Mueller et al. 2022, sec. 4: with a table much larger
+// than the batch most entries are untouched), and rewrites their fp16 copies.
+// The LDS sums take a parameter's contributions in no fixed order.
+constexpr uint32_t kPartEntries = 8192;  // 64 KB of LDS (two feature sums per entry)
+constexpr uint32_t kGridBlock = 1024;
+
+struct GridParts {
+    uint32_t first[kLevels + 1];  // first workgroup of each level; first[kLevels] = all workgroups
+    uint32_t entries[kLevels];    // table entries a level can touch
+    uint32_t chunks[kLevels];     // batch chunks of each level (workgroups = ranges x chunks)
+    uint32_t scratch[kLevels];    // chunked levels: first scratch entry ([chunk][entry] float2)
+    uint32_t finish[kLevels + 1]; // chunked levels: first entry index in nrc_grid_finish_kernel's range
+};
+
+struct GridUpdate {
+    AdamStep a;
+    float *w, *m, *v, *grad;
+    __half *half_w;
+    int mode;
+};
+
+__device__ __forceinline__ void finish_entry(const GridUpdate &u, size_t j, float2 gs)
+{
+    if (u.mode == kGradient) {
+        u.grad[j] = gs.x;
+        u.grad[j + 1u] = gs.y;
+        return;
     }
-    half_w[j] = __float2half_rn(p);
+    const float gv[2] = {gs.x, gs.y};
+    for (uint32_t f = 0; f < 2u; f++) {
+        if (gv[f] == 0.0f) continue;
+        float mj = u.m[j + f], vj = u.v[j + f];
+        const float p = adam(u.w[j + f], gv[f], mj, vj, u.a);
+        u.m[j + f] = mj;
+        u.v[j + f] = vj;
+        u.w[j + f] = p;
+        u.half_w[j + f] = __float2half_rn(p);
+    }
+}
+
+__global__ __launch_bounds__(kGridBlock) void nrc_grid_kernel(NrcGrid g, GridParts parts,
+                                                              const float4 *__restrict__ sample_u,
+                                                              const float2 *__restrict__ sample_dx, uint32_t dx_stride,
+                                                              const uint32_t *__restrict__ count_ptr, uint32_t count_max,
+                                                              uint32_t batch, float2 *__restrict__ scratch, GridUpdate up)
+{
+    __shared__ float2 s_acc[kPartEntries];
+    uint32_t l = 0;
+    while (l + 1u < kLevels && blockIdx.x >= parts.first[l + 1u]) l++;
+    const uint32_t rel = blockIdx.x - parts.first[l], chunks = parts.chunks[l];
+    const uint32_t e0 = (rel / chunks) * kPartEntries, chunk = rel % chunks;
+    const uint32_t span = min(parts.entries[l] - e0, kPartEntries);
+    for (uint32_t i = threadIdx.x; i < kPartEntries; i += kGridBlock) s_acc[i] = make_float2(0.0f, 0.0f);
+    __syncthreads();
+    const uint32_t total = min(count_ptr ? *count_ptr : count_max, count_max);
+    const uint32_t n_batch = total ? batch : 0u;
+    const uint32_t per = (n_batch + chunks - 1u) / chunks;
+    const uint32_t b_end = min(n_batch, (chunk + 1u) * per);
+    const float2 *dx = sample_dx + size_t(l) * dx_stride;
+    for (uint32_t b = chunk * per + threadIdx.x; b < b_end; b += kGridBlock) {
+        const float2 d = dx[b];
+        if (d.x == 0.0f && d.y == 0.0f) continue;
+        const float4 uu = sample_u[b];
+        const float u[3] = {uu.x, uu.y, uu.z};
+        uint32_t idx[8];
+        float wc[8];
+        level_corners(g, l, u, idx, wc);
+        for (int c = 0; c < 8; c++) {
+            const uint32_t e = idx[c] - e0;
+            if (e < span) {
+                if (d.x != 0.0f) atomicAdd(&s_acc[e].x, wc[c] * d.x);
+                if (d.y != 0.0f) atomicAdd(&s_acc[e].y, wc[c] * d.y);
+            }
+        }
+    }
+    __syncthreads();
+    if (chunks > 1u) {
+        float2 *dst = scratch + parts.scratch[l] + size_t(chunk) * parts.entries[l] + e0;
+        for (uint32_t i = threadIdx.x; i < span; i += kGridBlock) dst[i] = s_acc[i];
+        return;
+    }
+    const size_t base = ((size_t(l) << g.log2_table) + e0) * 2u;  // parameter index of the range's first entry
+    for (uint32_t i = threadIdx.x; i < span; i += kGridBlock) finish_entry(up, base + 2u * i, s_acc[i]);
+}
+
+// The chunked levels: one entry per thread, its chunk sums added in order.
+__global__ void nrc_grid_finish_kernel(NrcGrid g, GridParts parts, const float2 *__restrict__ scratch, GridUpdate up)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= parts.finish[kLevels]) return;
+    uint32_t l = 0;
+    while (t >= parts.finish[l + 1u]) l++;  // parts.finish[l + 1] > t for some l < kLevels
+    const uint32_t e = t - parts.finish[l], chunks = parts.chunks[l], entries = parts.entries[l];
+    const float2 *src = scratch + parts.scratch[l] + e;
+    float2 gs = src[0];
+    for (uint32_t c = 1; c < chunks; c++) {
+        const float2 x = src[size_t(c) * entries];
+        gs.x = gs.x + x.x;
+        gs.y = gs.y + x.y;
+    }
+    finish_entry(up, ((size_t(l) << g.log2_table) + e) * 2u, gs);
 }
 
 __global__ void nrc_zero_kernel(float *a, uint64_t n)
@@ -839,6 +955,10 @@ struct orn_nrc {
     __half *grid_half;
     float *partials;  // per training workgroup weight gradients
     uint32_t partial_groups;
+    float4 *sample_u;   // per batch sample: unit position
+    float2 *sample_dx;  // per level and batch sample: loss gradient of the two grid features
+    GridParts parts;    // nrc_grid_kernel's workgroups per level
+    float2 *grid_scratch;  // chunk sums of the chunked levels
     NrcRecord *train_recs;
     uint32_t train_cap;
     NrcRecord *queries;
@@ -893,7 +1013,7 @@ void free_nrc(orn_nrc *n)
     (void)hipStreamSynchronize(n->scene->ctx->stream);
     void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
                     n->grid_grad, n->grid_half, n->partials, n->train_recs, n->queries, n->partial, n->counters,
-                    n->loss};
+                    n->loss, n->sample_u, n->sample_dx, n->grid_scratch};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : n->ev)
@@ -921,29 +1041,45 @@ AdamStep adam_step(const orn_nrc *n)
     return a;
 }
 
-// Working copies from the masters (update = 0) or an Adam step (update = 1).
-hipError_t apply_params(orn_nrc *n, int update, hipStream_t st)
+// Working copies (bf16 MLP, fp16 grid) from the masters.
+hipError_t refresh_params(orn_nrc *n, hipStream_t st)
 {
     const AdamStep a = adam_step(n);
-    nrc_adam_mlp_kernel<<<blocks_for(kMlpParams, 256), 256, 0, st>>>(a, n->mlp, n->mlp_grad, n->mlp_m, n->mlp_v,
-                                                                     n->mlp_wt, update);
+    nrc_mlp_kernel<<<kMlpParams / 64u, kBlock, 0, st>>>(nullptr, 0u, a, n->mlp, n->mlp_grad, n->mlp_m, n->mlp_v,
+                                                       n->mlp_wt, kRefresh);
     const uint64_t g = n->grid_params;
-    nrc_adam_grid_kernel<<<static_cast<uint32_t>((g + 255) / 256), 256, 0, st>>>(a, n->grid_w, n->grid_grad, n->grid_m,
-                                                                                n->grid_v, n->grid_half, g, update);
+    nrc_grid_half_kernel<<<static_cast<uint32_t>((g + 255) / 256), 256, 0, st>>>(n->grid_w, n->grid_half, g);
     return hipGetLastError();
 }
 
 // Forward + backward over records [first, first + batch) of buf (count from
-// the device counter or count_max), weight gradient reduced into mlp_grad.
+// the device counter or count_max), then (mode) the gradients stored into
+// mlp_grad and grid_grad, or an Adam step with step count n->step.
 hipError_t gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr, uint32_t count_max, uint32_t first,
-                     uint32_t batch, hipStream_t st)
+                     uint32_t batch, GradMode mode, hipStream_t st)
 {
     const uint32_t groups = blocks_for(batch, kTileRows);
     if (groups > n->partial_groups) return hipErrorInvalidValue;
     nrc_train_kernel<<<groups, kBlock, 0, st>>>(n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt,
-                                                buf, count_ptr, count_max, first, batch, n->partials, n->grid_grad,
-                                                n->loss);
-    nrc_reduce_kernel<<<blocks_for(kMlpParams, 256), 256, 0, st>>>(n->partials, groups, n->mlp_grad);
+                                                buf, count_ptr, count_max, first, batch, n->partials, n->sample_u,
+                                                n->sample_dx, n->train_cap, n->loss);
+    const AdamStep a = adam_step(n);
+    nrc_mlp_kernel<<<kMlpParams / 64u, kBlock, 0, st>>>(n->partials, groups, a, n->mlp, n->mlp_grad, n->mlp_m,
+                                                       n->mlp_v, n->mlp_wt, mode);
+    GridUpdate up;
+    up.a = a;
+    up.w = n->grid_w;
+    up.m = n->grid_m;
+    up.v = n->grid_v;
+    up.grad = n->grid_grad;
+    up.half_w = n->grid_half;
+    up.mode = mode;
+    nrc_grid_kernel<<<n->parts.first[kLevels], kGridBlock, 0, st>>>(n->grid, n->parts, n->sample_u, n->sample_dx,
+                                                                    n->train_cap, count_ptr, count_max, batch,
+                                                                    n->grid_scratch, up);
+    if (n->parts.finish[kLevels])
+        nrc_grid_finish_kernel<<<blocks_for(n->parts.finish[kLevels], 256), 256, 0, st>>>(n->grid, n->parts,
+                                                                                        n->grid_scratch, up);
     return hipGetLastError();
 }
 
@@ -1008,6 +1144,20 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         g.res[l] = std::max<uint32_t>(1u, static_cast<uint32_t>(std::floor(cfg->base_resolution * std::pow(growth, l))));
         const uint64_t r1 = g.res[l] + 1ull;
         g.dense[l] = r1 * r1 * r1 <= table ? 1u : 0u;
+        n->parts.entries[l] = g.dense[l] ? static_cast<uint32_t>(r1 * r1 * r1) : static_cast<uint32_t>(table);
+    }
+    // nrc_grid_kernel: a level with fewer entry ranges than the largest
+    // level splits the batch into as many more chunks (at most 64)
+    GridParts &gp_ = n->parts;
+    const uint32_t most = blocks_for(static_cast<uint32_t>(table), kPartEntries);
+    uint64_t scratch_entries = 0;
+    for (uint32_t l = 0; l < kLevels; l++) {
+        const uint32_t ranges = blocks_for(gp_.entries[l], kPartEntries);
+        gp_.chunks[l] = std::min<uint32_t>(64u, std::max<uint32_t>(1u, most / ranges));
+        gp_.first[l + 1u] = gp_.first[l] + ranges * gp_.chunks[l];
+        gp_.scratch[l] = static_cast<uint32_t>(scratch_entries);
+        gp_.finish[l + 1u] = gp_.finish[l] + (gp_.chunks[l] > 1u ? gp_.entries[l] : 0u);
+        if (gp_.chunks[l] > 1u) scratch_entries += uint64_t(gp_.chunks[l]) * gp_.entries[l];
     }
     for (int a = 0; a < 3; a++) {
         float lo = scene->bounds_lo[a], hi = scene->bounds_hi[a];
@@ -1034,6 +1184,9 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         (e = hipMalloc(&n->grid_grad, gp * 4)) || (e = hipMalloc(&n->grid_half, gp * 2)) ||
         (e = hipMalloc(&n->partials, size_t(n->partial_groups) * kMlpParams * 4)) ||
         (e = hipMalloc(&n->train_recs, size_t(n->train_cap) * sizeof(NrcRecord))) ||
+        (e = hipMalloc(&n->sample_u, size_t(n->train_cap) * sizeof(float4))) ||
+        (e = hipMalloc(&n->sample_dx, size_t(n->train_cap) * kLevels * sizeof(float2))) ||
+        (scratch_entries && (e = hipMalloc(&n->grid_scratch, size_t(scratch_entries) * sizeof(float2)))) ||
         (e = hipMalloc(&n->counters, 16)) || (e = hipMalloc(&n->loss, 16)))
         return fail(e);
     for (hipEvent_t &ev : n->ev)
@@ -1063,7 +1216,7 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
     nrc_zero_kernel<<<zero_blocks, 256, 0, st>>>(n->grid_grad, gp);
     nrc_zero_kernel<<<blocks_for(kWtElems / 2u, 256), 256, 0, st>>>(reinterpret_cast<float *>(n->mlp_wt),
                                                                     kWtElems / 2u);  // row padding stays 0
-    if ((e = apply_params(n, 0, st)) || (e = hipStreamSynchronize(st))) return fail(e);
+    if ((e = refresh_params(n, st)) || (e = hipStreamSynchronize(st))) return fail(e);
     *out = n;
     return ORN_OK;
 }
@@ -1116,7 +1269,7 @@ orn_status orn_nrc_set_params(orn_nrc *n, const float *mlp, size_t mlp_len, cons
     hipStream_t st = n->scene->ctx->stream;
     if (mlp) ORN_TRY(hipMemcpyAsync(n->mlp, mlp, kMlpParams * 4, hipMemcpyHostToDevice, st));
     if (grid) ORN_TRY(hipMemcpyAsync(n->grid_w, grid, size_t(n->grid_params) * 4, hipMemcpyHostToDevice, st));
-    ORN_TRY(apply_params(n, 0, st));
+    ORN_TRY(refresh_params(n, st));
     ORN_TRY(hipStreamSynchronize(st));
     return ORN_OK;
 }
@@ -1162,16 +1315,17 @@ static orn_status host_batch(orn_nrc *n, const orn_nrc_record *records, size_t c
     const uint32_t c = static_cast<uint32_t>(count);
     ORN_TRY(hipMemcpyAsync(n->train_recs, records, count * sizeof(NrcRecord), hipMemcpyHostToDevice, st));
     nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
-    ORN_TRY(gradients(n, n->train_recs, nullptr, c, 0u, c, st));
-    if (mlp_grad) ORN_TRY(hipMemcpyAsync(mlp_grad, n->mlp_grad, kMlpParams * 4, hipMemcpyDeviceToHost, st));
-    if (grid_grad)
-        ORN_TRY(hipMemcpyAsync(grid_grad, n->grid_grad, size_t(n->grid_params) * 4, hipMemcpyDeviceToHost, st));
     if (update) {
         n->step++;
-        ORN_TRY(apply_params(n, 1, st));
+        ORN_TRY(gradients(n, n->train_recs, nullptr, c, 0u, c, kUpdate, st));
     } else {
+        // entries of dense levels past their lattice are never written by the gradient kernel
         const uint64_t gp = n->grid_params;
         nrc_zero_kernel<<<static_cast<uint32_t>((gp + 255) / 256), 256, 0, st>>>(n->grid_grad, gp);
+        ORN_TRY(gradients(n, n->train_recs, nullptr, c, 0u, c, kGradient, st));
+        if (mlp_grad) ORN_TRY(hipMemcpyAsync(mlp_grad, n->mlp_grad, kMlpParams * 4, hipMemcpyDeviceToHost, st));
+        if (grid_grad)
+            ORN_TRY(hipMemcpyAsync(grid_grad, n->grid_grad, size_t(n->grid_params) * 4, hipMemcpyDeviceToHost, st));
     }
     float l = 0.0f;
     ORN_TRY(hipMemcpyAsync(&l, n->loss, 4, hipMemcpyDeviceToHost, st));
@@ -1255,9 +1409,8 @@ orn_status enqueue_steps(orn_nrc *n, uint32_t steps)
     for (uint32_t k = 0; k < steps; k++) {
         // step k takes records k*b, k*b + 1, ... of this frame's samples,
         // modulo their count (an empty frame makes a step with zero gradient)
-        ORN_TRY(gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
         n->step++;
-        ORN_TRY(apply_params(n, 1, st));
+        ORN_TRY(gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, kUpdate, st));
     }
     return ORN_OK;
 }
