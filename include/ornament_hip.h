@@ -347,7 +347,9 @@ orn_status orn_batch_autotune(orn_batch *batch, const orn_constant_params *param
  * terminate_bounce and, on a scattering surface, replaces the rest of the
  * path with throughput x prediction. orn_nrc_train traces train_paths paths
  * with their own RNG streams to full depth, records the inputs and the
- * estimator's value at that vertex, and takes Adam steps (relative L2 loss). */
+ * estimator's value at that vertex, and takes Adam steps (relative L2 loss;
+ * hash-grid parameters whose gradient in a step is exactly 0 skip that step,
+ * keeping value and moments, as in Instant-NGP). */
 typedef struct orn_nrc orn_nrc;
 typedef struct orn_nrc_config {
     uint32_t log2_table;       /* default 19 */

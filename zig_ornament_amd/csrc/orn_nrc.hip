@@ -610,19 +610,27 @@ __global__ __launch_bounds__(kGridBlock) void nrc_grid_kernel(NrcGrid g, GridPar
     const uint32_t per = (n_batch + chunks - 1u) / chunks;
     const uint32_t b_end = min(n_batch, (chunk + 1u) * per);
     const float2 *dx = sample_dx + size_t(l) * dx_stride;
-    for (uint32_t b = chunk * per + threadIdx.x; b < b_end; b += kGridBlock) {
-        const float2 d = dx[b];
-        if (d.x == 0.0f && d.y == 0.0f) continue;
-        const float4 uu = sample_u[b];
-        const float u[3] = {uu.x, uu.y, uu.z};
-        uint32_t idx[8];
-        float wc[8];
-        level_corners(g, l, u, idx, wc);
-        for (int c = 0; c < 8; c++) {
-            const uint32_t e = idx[c] - e0;
-            if (e < span) {
-                if (d.x != 0.0f) atomicAdd(&s_acc[e].x, wc[c] * d.x);
-                if (d.y != 0.0f) atomicAdd(&s_acc[e].y, wc[c] * d.y);
+    constexpr uint32_t kAhead = 4;  // samples per thread whose loads are in flight together
+    for (uint32_t b0 = chunk * per + threadIdx.x; b0 < b_end; b0 += kAhead * kGridBlock) {
+        float2 d[kAhead];
+        float4 uu[kAhead];
+        for (uint32_t k = 0; k < kAhead; k++) {
+            const uint32_t b = b0 + k * kGridBlock;
+            d[k] = b < b_end ? dx[b] : make_float2(0.0f, 0.0f);
+            uu[k] = b < b_end ? sample_u[b] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        for (uint32_t k = 0; k < kAhead; k++) {
+            if (d[k].x == 0.0f && d[k].y == 0.0f) continue;
+            const float u[3] = {uu[k].x, uu[k].y, uu[k].z};
+            uint32_t idx[8];
+            float wc[8];
+            level_corners(g, l, u, idx, wc);
+            for (int c = 0; c < 8; c++) {
+                const uint32_t e = idx[c] - e0;
+                if (e < span) {
+                    if (d[k].x != 0.0f) atomicAdd(&s_acc[e].x, wc[c] * d[k].x);
+                    if (d[k].y != 0.0f) atomicAdd(&s_acc[e].y, wc[c] * d[k].y);
+                }
             }
         }
     }
@@ -1147,13 +1155,13 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         n->parts.entries[l] = g.dense[l] ? static_cast<uint32_t>(r1 * r1 * r1) : static_cast<uint32_t>(table);
     }
     // nrc_grid_kernel: a level with fewer entry ranges than the largest
-    // level splits the batch into as many more chunks (at most 64)
+    // level splits the batch into as many more chunks (at most 16)
     GridParts &gp_ = n->parts;
     const uint32_t most = blocks_for(static_cast<uint32_t>(table), kPartEntries);
     uint64_t scratch_entries = 0;
     for (uint32_t l = 0; l < kLevels; l++) {
         const uint32_t ranges = blocks_for(gp_.entries[l], kPartEntries);
-        gp_.chunks[l] = std::min<uint32_t>(64u, std::max<uint32_t>(1u, most / ranges));
+        gp_.chunks[l] = std::min<uint32_t>(16u, std::max<uint32_t>(1u, most / ranges));
         gp_.first[l + 1u] = gp_.first[l] + ranges * gp_.chunks[l];
         gp_.scratch[l] = static_cast<uint32_t>(scratch_entries);
         gp_.finish[l + 1u] = gp_.finish[l] + (gp_.chunks[l] > 1u ? gp_.entries[l] : 0u);
