@@ -862,11 +862,16 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)  // lowbias32 (Wellons)
 // at the hit of bounce k on a scattering surface the inputs are recorded and
 // the path continues with the reference estimator; the record's target is
 // the value of the path from that vertex on (throughput 1 at the vertex).
+// Path j leaves its record in stage[j] with keep[j] = 1 (else keep[j] = 0),
+// and each workgroup its record count in block_keep; nrc_compact_kernel then
+// packs the records in path order, so a frame's samples do not depend on
+// the order in which waves finish.
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
                                                                  uint32_t n_paths, uint32_t frame,
-                                                                 NrcRecord *__restrict__ recs,
-                                                                 uint32_t *__restrict__ rec_count)
+                                                                 NrcRecord *__restrict__ stage,
+                                                                 uint32_t *__restrict__ keep_flags,
+                                                                 uint32_t *__restrict__ block_keep)
 {
     extern __shared__ uint32_t lds_stack[];
     uint32_t *stack = lds_stack + threadIdx.x;
@@ -900,9 +905,55 @@ __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, or
             if (!scatter<F>(sc, s, ray, rng, next, w)) break;
             ray = next;
         }
+        keep_flags[j] = keep ? 1u : 0u;
+        if (keep) stage[j] = rc;
     }
-    const uint32_t slot = wave_append(keep, rec_count);
-    if (keep) recs[slot] = rc;
+    __shared__ uint32_t s_kept[kBlock / 64u];
+    const uint64_t m = __ballot(keep);
+    if (threadIdx.x % 64u == 0u) s_kept[threadIdx.x / 64u] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        uint32_t c = 0;
+        for (uint32_t w = 0; w < kBlock / 64u; w++) c += s_kept[w];
+        block_keep[blockIdx.x] = c;
+    }
+}
+
+// Packs the kept training records in path order: workgroup b (the paths of
+// nrc_train_paths_kernel's workgroup b) starts after the records of
+// workgroups 0 .. b-1; the last workgroup stores the total.
+__global__ __launch_bounds__(kBlock) void nrc_compact_kernel(const NrcRecord *__restrict__ stage,
+                                                             const uint32_t *__restrict__ keep_flags,
+                                                             const uint32_t *__restrict__ block_keep,
+                                                             uint32_t n_paths, NrcRecord *__restrict__ recs,
+                                                             uint32_t *__restrict__ rec_count)
+{
+    __shared__ uint32_t s_part[kBlock / 64u];
+    __shared__ uint32_t s_wave[kBlock / 64u];
+    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    uint32_t before = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kBlock) before += block_keep[b];
+    for (int off = 32; off > 0; off >>= 1) before += __shfl_xor(before, off);
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const bool keep = j < n_paths && keep_flags[j] != 0u;
+    const uint64_t m = __ballot(keep);
+    if (lane == 0u) {
+        s_part[wave] = before;
+        s_wave[wave] = static_cast<uint32_t>(__popcll(m));
+    }
+    __syncthreads();
+    uint32_t base = 0, in_block = 0;
+    for (uint32_t w = 0; w < kBlock / 64u; w++) {
+        base += s_part[w];
+        if (w < wave) in_block += s_wave[w];
+    }
+    const uint32_t slot = base + in_block + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+    if (keep) recs[slot] = stage[j];
+    if (blockIdx.x + 1u == gridDim.x && threadIdx.x == 0u) {
+        uint32_t total = base;
+        for (uint32_t w = 0; w < kBlock / 64u; w++) total += s_wave[w];
+        *rec_count = total;
+    }
 }
 
 // render() bookkeeping of one sample per pixel (pathtracer.wgsl:64-79):
@@ -968,6 +1019,9 @@ struct orn_nrc {
     GridParts parts;    // nrc_grid_kernel's workgroups per level
     float2 *grid_scratch;  // chunk sums of the chunked levels
     NrcRecord *train_recs;
+    NrcRecord *train_stage;      // per training path (nrc_train_paths_kernel)
+    uint32_t *train_keep;        // per training path: 1 if it left a record
+    uint32_t *train_block_keep;  // records per training-path workgroup
     uint32_t train_cap;
     NrcRecord *queries;
     float4 *partial;
@@ -999,8 +1053,11 @@ template <uint32_t F>
 void launch_train_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, uint32_t frame, hipStream_t st,
                         size_t lds)
 {
-    nrc_train_paths_kernel<F><<<blocks_for(n->cfg.train_paths, kBlock), kBlock, lds, st>>>(
-        ds, p, n->cfg.terminate_bounce, n->cfg.train_paths, frame, n->train_recs, n->counters + 1);
+    const uint32_t blocks = blocks_for(n->cfg.train_paths, kBlock);
+    nrc_train_paths_kernel<F><<<blocks, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, n->cfg.train_paths, frame,
+                                                          n->train_stage, n->train_keep, n->train_block_keep);
+    nrc_compact_kernel<<<blocks, kBlock, 0, st>>>(n->train_stage, n->train_keep, n->train_block_keep,
+                                                  n->cfg.train_paths, n->train_recs, n->counters + 1);
 }
 
 #define ORN_NRC_DISPATCH(fn, features, ...)        \
@@ -1021,7 +1078,8 @@ void free_nrc(orn_nrc *n)
     (void)hipStreamSynchronize(n->scene->ctx->stream);
     void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
                     n->grid_grad, n->grid_half, n->partials, n->train_recs, n->queries, n->partial, n->counters,
-                    n->loss, n->sample_u, n->sample_dx, n->grid_scratch};
+                    n->loss, n->sample_u, n->sample_dx, n->grid_scratch, n->train_stage, n->train_keep,
+                    n->train_block_keep};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : n->ev)
@@ -1192,6 +1250,9 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         (e = hipMalloc(&n->grid_grad, gp * 4)) || (e = hipMalloc(&n->grid_half, gp * 2)) ||
         (e = hipMalloc(&n->partials, size_t(n->partial_groups) * kMlpParams * 4)) ||
         (e = hipMalloc(&n->train_recs, size_t(n->train_cap) * sizeof(NrcRecord))) ||
+        (e = hipMalloc(&n->train_stage, size_t(std::max(cfg->train_paths, 1u)) * sizeof(NrcRecord))) ||
+        (e = hipMalloc(&n->train_keep, size_t(std::max(cfg->train_paths, 1u)) * sizeof(uint32_t))) ||
+        (e = hipMalloc(&n->train_block_keep, size_t(blocks_for(std::max(cfg->train_paths, 1u), kBlock)) * 4u)) ||
         (e = hipMalloc(&n->sample_u, size_t(n->train_cap) * sizeof(float4))) ||
         (e = hipMalloc(&n->sample_dx, size_t(n->train_cap) * kLevels * sizeof(float2))) ||
         (scratch_entries && (e = hipMalloc(&n->grid_scratch, size_t(scratch_entries) * sizeof(float2)))) ||
