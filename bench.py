@@ -321,7 +321,8 @@ def main():
                            "traffic": profiled_traffic(),
                            "algorithmic_bytes_per_sample": cb["bytes_per_sample"],
                            "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
-                           "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"]}
+                           "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"],
+                           "rays_per_s": value * cb["stats"]["rays"] / cb["stats"]["samples"]}
         out["cpu_baseline"] = {"value": cb["samples_per_s"], "unit": "samples/s",
                                "cores": cb["threads"], "kind": "port",
                                "sample": f"{cb['pixels']} evenly spaced pixels of the same image "
