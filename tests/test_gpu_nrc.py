@@ -118,6 +118,31 @@ def test_gradients_match_oracle_and_autograd(cache):
     assert np.all(gg[np.setdiff1d(np.arange(gg.size), touched)] == 0)
 
 
+def test_grid_gradient_at_full_table_size(device):
+    """2^19-entry tables, as the 120-frame workload uses them: five densely
+    indexed coarse levels whose batch is cut into up to 16 chunks (summed by
+    nrc_grid_finish_kernel) and eleven hashed levels of 64 entry ranges each."""
+    spec, ds = scene(device, seed=36)
+    nrc = Nrc(ds, log2_table=19, train_batch=4096, train_paths=4096)
+    mlp, grid = random_params(nrc, 11, grid_scale=0.05)
+    info = nrc.info()
+    assert info["dense"][:5] == [1] * 5 and not any(info["dense"][5:])
+    rec = records(info, 4096, 6)
+    loss, mg, gg = nrc.gradients(rec)
+    x = R.inputs(info, R.grid_table(info, grid), rec)
+    xs, y = R.forward(mlp, x)
+    _, gy = R.loss_and_grad(y, rec["weight"].astype(np.float64))
+    g, dx0 = R.backward(mlp, xs, gy)
+    assert np.linalg.norm(mg - g) <= 1e-2 * np.linalg.norm(g)
+    ggo = R.grid_gradient(info, rec["pos"], dx0[:, :32])
+    t = 1 << info["log2_table"]
+    for l in range(16):
+        sl = slice(l * 2 * t, (l + 1) * 2 * t)
+        a, b = gg[sl], ggo[sl]
+        assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b), l
+        assert np.all(a[b == 0] == 0), l
+
+
 def test_adam_steps_match_oracle(device):
     """Two Adam steps on different batches: the MLP takes dense steps; grid
     parameters with an exactly-zero gradient in the second batch keep the
