@@ -266,3 +266,39 @@ def test_frames_render_and_train_like_separate_calls(device):
     rel = np.linalg.norm(ga - gb) / np.linalg.norm(gb)
     print(f"[nrc] grid difference after 4 steps: {rel:.2e} (relative norm)", flush=True)
     assert rel <= 1e-6
+
+
+def test_cache_rejects_bad_arguments(device):
+    """Configurations and calls outside the documented ranges return a status
+    and leave the cache usable."""
+    spec, ds = scene(device, seed=37)
+    other_spec, other = scene(device, seed=38)
+
+    INVALID_VALUE, INVALID_HANDLE, SIZE_MISMATCH = 1, 9006, 9007  # include/ornament_hip.h
+
+    def status(fn):
+        with pytest.raises(zo.OrnamentError) as e:
+            fn()
+        return e.value.status
+
+    for bad in (dict(train_batch=100), dict(train_batch=0), dict(log2_table=7), dict(log2_table=25),
+                dict(base_resolution=1), dict(max_resolution=8, base_resolution=16),
+                dict(learning_rate=0.0), dict(beta1=1.0), dict(epsilon=-1.0)):
+        assert status(lambda: Nrc(ds, **bad)) == INVALID_VALUE, bad
+    nrc = Nrc(ds, log2_table=10, train_batch=256, train_paths=256)
+    p = zo_params(spec)
+    t_other = zo.Target(other, spec.width, spec.height)
+    assert status(lambda: nrc.render(t_other, p, 1)) == INVALID_HANDLE  # a target of another scene
+    t_small = zo.Target(ds, spec.width // 2, spec.height)
+    assert status(lambda: nrc.render(t_small, p, 1)) == SIZE_MISMATCH
+    mlp, grid = nrc.get_params()
+    assert status(lambda: nrc.set_params(mlp[:-1], None)) == SIZE_MISMATCH
+    assert status(lambda: nrc.set_params(None, grid[:-1])) == SIZE_MISMATCH
+    rec = records(nrc.info(), 300, 7)  # more records than the cache's training capacity (256)
+    assert status(lambda: nrc.step(rec)) == INVALID_VALUE
+    assert status(lambda: nrc.query(rec[:0])) == INVALID_VALUE
+    # still usable
+    t = zo.Target(ds, spec.width, spec.height)
+    nrc.render(t, p, 1)
+    assert np.isfinite(nrc.step(rec[:256]))
+    assert nrc.info()["adam_steps"] == 1
