@@ -212,6 +212,9 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
         within = [q for q in plain if q["ms"] <= r["ms_per_frame"]]
         match = next((q for q in plain if q["rmse"] <= r["rmse"]), None)
         r["plain_at_equal_time"] = within[-1] if within else None
+        if not within:
+            r["plain_at_equal_time_note"] = (f"one plain sample per pixel takes {plain[0]['ms']:.2f} ms, "
+                                             f"longer than the cache's frame")
         r["plain_to_match_rmse"] = match
         r["plain_time_over_nrc_time"] = match["ms"] / r["ms_per_frame"] if match else None
         out[name] = r
