@@ -302,3 +302,18 @@ def test_cache_rejects_bad_arguments(device):
     nrc.render(t, p, 1)
     assert np.isfinite(nrc.step(rec[:256]))
     assert nrc.info()["adam_steps"] == 1
+
+
+def test_training_records_pack_in_path_order_at_scale(device):
+    """300,000 training paths (1,172 path workgroups, more than the scan
+    workgroup's 1,024 lanes, so each lane sums a run of counts): two caches
+    take the same step to the same MLP weights bit for bit, which they can
+    only do if both packed the same records in the same order."""
+    spec, ds = scene(device, seed=39)
+    p = zo_params(spec)
+    cfg = dict(log2_table=12, terminate_bounce=0, train_paths=300000, train_batch=65536)
+    a, b = Nrc(ds, **cfg), Nrc(ds, **cfg)
+    la = a.train(p, 5, steps=1)
+    lb = b.train(p, 5, steps=1)
+    assert np.isfinite(la) and la > 0 and abs(la - lb) <= 1e-5 * la
+    assert np.array_equal(a.get_params()[0], b.get_params()[0])

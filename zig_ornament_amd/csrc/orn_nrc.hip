@@ -863,9 +863,9 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)  // lowbias32 (Wellons)
 // the path continues with the reference estimator; the record's target is
 // the value of the path from that vertex on (throughput 1 at the vertex).
 // Path j leaves its record in stage[j] with keep[j] = 1 (else keep[j] = 0),
-// and each workgroup its record count in block_keep; nrc_compact_kernel then
-// packs the records in path order, so a frame's samples do not depend on
-// the order in which waves finish.
+// and each workgroup its record count in block_keep; nrc_scan_kernel and
+// nrc_compact_kernel then pack the records in path order, so a frame's
+// samples do not depend on the order in which waves finish.
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
                                                                  uint32_t n_paths, uint32_t frame,
@@ -919,41 +919,53 @@ __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, or
     }
 }
 
-// Packs the kept training records in path order: workgroup b (the paths of
-// nrc_train_paths_kernel's workgroup b) starts after the records of
-// workgroups 0 .. b-1; the last workgroup stores the total.
+// Exclusive prefix sums of the per-workgroup record counts (one workgroup of
+// kScanBlock lanes, each summing a contiguous run of counts): offsets[b] is
+// the first record slot of training-path workgroup b; *rec_count the total.
+constexpr uint32_t kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void nrc_scan_kernel(const uint32_t *__restrict__ block_keep, uint32_t blocks,
+                                                              uint32_t *__restrict__ offsets,
+                                                              uint32_t *__restrict__ rec_count)
+{
+    __shared__ uint32_t s_sum[kScanBlock];
+    const uint32_t per = (blocks + kScanBlock - 1u) / kScanBlock;
+    const uint32_t b0 = min(threadIdx.x * per, blocks), b1 = min(b0 + per, blocks);
+    uint32_t run = 0;
+    for (uint32_t b = b0; b < b1; b++) run += block_keep[b];
+    s_sum[threadIdx.x] = run;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanBlock; off <<= 1) {  // inclusive scan (Hillis-Steele)
+        const uint32_t v = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_sum[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t at = s_sum[threadIdx.x] - run;
+    for (uint32_t b = b0; b < b1; b++) {
+        offsets[b] = at;
+        at += block_keep[b];
+    }
+    if (threadIdx.x == kScanBlock - 1u) *rec_count = s_sum[kScanBlock - 1u];
+}
+
+// Packs the kept training records in path order: the paths of
+// nrc_train_paths_kernel's workgroup b start at slot offsets[b].
 __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(const NrcRecord *__restrict__ stage,
                                                              const uint32_t *__restrict__ keep_flags,
-                                                             const uint32_t *__restrict__ block_keep,
-                                                             uint32_t n_paths, NrcRecord *__restrict__ recs,
-                                                             uint32_t *__restrict__ rec_count)
+                                                             const uint32_t *__restrict__ offsets,
+                                                             uint32_t n_paths, NrcRecord *__restrict__ recs)
 {
-    __shared__ uint32_t s_part[kBlock / 64u];
     __shared__ uint32_t s_wave[kBlock / 64u];
     const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
-    uint32_t before = 0;
-    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kBlock) before += block_keep[b];
-    for (int off = 32; off > 0; off >>= 1) before += __shfl_xor(before, off);
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     const bool keep = j < n_paths && keep_flags[j] != 0u;
     const uint64_t m = __ballot(keep);
-    if (lane == 0u) {
-        s_part[wave] = before;
-        s_wave[wave] = static_cast<uint32_t>(__popcll(m));
-    }
+    if (lane == 0u) s_wave[wave] = static_cast<uint32_t>(__popcll(m));
     __syncthreads();
-    uint32_t base = 0, in_block = 0;
-    for (uint32_t w = 0; w < kBlock / 64u; w++) {
-        base += s_part[w];
-        if (w < wave) in_block += s_wave[w];
-    }
-    const uint32_t slot = base + in_block + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+    uint32_t in_block = 0;
+    for (uint32_t w = 0; w < wave; w++) in_block += s_wave[w];
+    const uint32_t slot = offsets[blockIdx.x] + in_block + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
     if (keep) recs[slot] = stage[j];
-    if (blockIdx.x + 1u == gridDim.x && threadIdx.x == 0u) {
-        uint32_t total = base;
-        for (uint32_t w = 0; w < kBlock / 64u; w++) total += s_wave[w];
-        *rec_count = total;
-    }
 }
 
 // render() bookkeeping of one sample per pixel (pathtracer.wgsl:64-79):
@@ -1022,6 +1034,7 @@ struct orn_nrc {
     NrcRecord *train_stage;      // per training path (nrc_train_paths_kernel)
     uint32_t *train_keep;        // per training path: 1 if it left a record
     uint32_t *train_block_keep;  // records per training-path workgroup
+    uint32_t *train_offsets;     // first record slot of each training-path workgroup
     uint32_t train_cap;
     NrcRecord *queries;
     float4 *partial;
@@ -1056,8 +1069,9 @@ void launch_train_paths(orn_nrc *n, const DevScene &ds, const orn_constant_param
     const uint32_t blocks = blocks_for(n->cfg.train_paths, kBlock);
     nrc_train_paths_kernel<F><<<blocks, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, n->cfg.train_paths, frame,
                                                           n->train_stage, n->train_keep, n->train_block_keep);
-    nrc_compact_kernel<<<blocks, kBlock, 0, st>>>(n->train_stage, n->train_keep, n->train_block_keep,
-                                                  n->cfg.train_paths, n->train_recs, n->counters + 1);
+    nrc_scan_kernel<<<1, kScanBlock, 0, st>>>(n->train_block_keep, blocks, n->train_offsets, n->counters + 1);
+    nrc_compact_kernel<<<blocks, kBlock, 0, st>>>(n->train_stage, n->train_keep, n->train_offsets, n->cfg.train_paths,
+                                                  n->train_recs);
 }
 
 #define ORN_NRC_DISPATCH(fn, features, ...)        \
@@ -1079,7 +1093,7 @@ void free_nrc(orn_nrc *n)
     void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
                     n->grid_grad, n->grid_half, n->partials, n->train_recs, n->queries, n->partial, n->counters,
                     n->loss, n->sample_u, n->sample_dx, n->grid_scratch, n->train_stage, n->train_keep,
-                    n->train_block_keep};
+                    n->train_block_keep, n->train_offsets};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : n->ev)
@@ -1253,6 +1267,7 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         (e = hipMalloc(&n->train_stage, size_t(std::max(cfg->train_paths, 1u)) * sizeof(NrcRecord))) ||
         (e = hipMalloc(&n->train_keep, size_t(std::max(cfg->train_paths, 1u)) * sizeof(uint32_t))) ||
         (e = hipMalloc(&n->train_block_keep, size_t(blocks_for(std::max(cfg->train_paths, 1u), kBlock)) * 4u)) ||
+        (e = hipMalloc(&n->train_offsets, size_t(blocks_for(std::max(cfg->train_paths, 1u), kBlock)) * 4u)) ||
         (e = hipMalloc(&n->sample_u, size_t(n->train_cap) * sizeof(float4))) ||
         (e = hipMalloc(&n->sample_dx, size_t(n->train_cap) * kLevels * sizeof(float2))) ||
         (scratch_entries && (e = hipMalloc(&n->grid_scratch, size_t(scratch_entries) * sizeof(float2)))) ||
