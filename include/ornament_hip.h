@@ -152,8 +152,8 @@ typedef struct orn_hit {
     uint32_t material_index;
     uint32_t node_type; /* 1 sphere, 2 mesh */
     uint32_t inverted_transform_id;
-    uint32_t tri_id; /* 3 * global triangle index */
-    float uv[2];
+    uint32_t tri_id; /* 3 * global triangle index (mesh hits; 0 for spheres) */
+    float uv[2];     /* barycentrics (mesh hits; 0 for spheres) */
 } orn_hit;
 
 typedef struct orn_ctx orn_ctx;
@@ -436,6 +436,13 @@ orn_status orn_nrc_step(orn_nrc *nrc, const orn_nrc_record *records, size_t n, f
  * (bvh_hit, bvh.wgsl:14-137). Host arrays in and out; blocks. */
 orn_status orn_trace_rays(orn_device_scene *scene, float t_min, const float *rays, size_t n,
                           orn_hit *out);
+/* The same closest hits (bit for bit, BVH2) with each wave's 64 consecutive
+ * rays traced together: the wave visits the union of their nodes in the
+ * reference's order, each ray testing only the nodes its own traversal
+ * visits. Fast when consecutive rays are coherent (a camera tile), slower
+ * for scattered rays. No counterpart in the reference. */
+orn_status orn_trace_rays_packet(orn_device_scene *scene, float t_min, const float *rays, size_t n,
+                                 orn_hit *out);
 
 /* Evaluates the device's FP32 transcendental functions (the kernels' stand-ins
  * for the WGSL builtins pow, sin, cos, acos, atan2) over host arrays:

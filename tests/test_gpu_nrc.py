@@ -205,6 +205,32 @@ def test_no_termination_is_the_plain_render(device):
     assert same_bits(a.framebuffer(), b.framebuffer())
 
 
+@pytest.mark.parametrize("name", ["cornell", "spheres", "instanced"])
+def test_primary_queries_trace_like_the_plain_render(device, name):
+    """With every surface an emitter no camera path becomes a query, so the
+    cache's primary-hit render (nrc_primary_kernel: 8 x 8 wave packets through
+    bvh_hit_packet) must equal the plain render bit for bit: camera rays,
+    closest hits, sky, emission (textured too), RNG state and accumulation.
+    Ragged image sizes leave lanes without a pixel in the last packets."""
+    import dataclasses
+    spec = {"cornell": W.cornell(width=45, height=37, spp=3),
+            "spheres": W.random_spheres(width=50, height=29, spp=3, grid=range(-4, 4)),
+            "instanced": W.instanced_icospheres(width=61, height=33, spp=3, levels=2, instances=12,
+                                                texture_size=16, box=10.0)}[name]
+    spec = dataclasses.replace(spec, materials=[("light", a if a is not None else (1.0, 1.0, 1.0), None)
+                                                for _, a, _ in spec.materials])
+    ds = zo.DeviceScene(device, zo.Bvh(W.build_scene(spec)))
+    nrc = Nrc(ds, log2_table=10, terminate_bounce=0, train_paths=0)
+    a = zo.Target(ds, spec.width, spec.height)
+    b = zo.Target(ds, spec.width, spec.height)
+    p = zo_params(spec)
+    nrc.render(a, p, spec.spp)
+    b.render(p, spec.spp)
+    assert same_bits(a.accumulation(), b.accumulation())
+    assert np.array_equal(a.rng_state(), b.rng_state())
+    assert same_bits(a.framebuffer(), b.framebuffer())
+
+
 def rmse(img, ref):
     return float(np.sqrt(np.mean((img[:, :3].astype(np.float64) - ref[:, :3]) ** 2)))
 

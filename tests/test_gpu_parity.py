@@ -50,6 +50,43 @@ def test_trace_rays_match_oracle(device, name, width):
     assert np.array_equal(g["uv"][m].view(np.uint32), o["uv"][m].view(np.uint32))
 
 
+def tiled_camera_rays(spec, width, height):
+    """Pinhole rays through the pixel centres of a width x height film, in
+    8 x 8 tiles (64 consecutive rays per tile, as a wave traces them)."""
+    cam = O.constant_params(spec).camera
+    o = np.array(cam.origin, np.float64)
+    llc = np.array(cam.lower_left_corner, np.float64)
+    h = np.array(cam.horizontal, np.float64)
+    v = np.array(cam.vertical, np.float64)
+    out = []
+    for ty in range(0, height, 8):
+        for tx in range(0, width, 8):
+            for k in range(64):
+                x, y = tx + k % 8, ty + k // 8
+                d = llc + (x + 0.5) / width * h + (y + 0.5) / height * v - o
+                out.append(np.concatenate([o, d]))
+    return np.asarray(out, np.float32)
+
+
+@pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
+def test_packet_traversal_matches_per_ray(device, name):
+    """orn_trace_rays_packet traces each wave's 64 rays together; every ray
+    must still get its own traversal's closest hit bit for bit, for coherent
+    camera tiles and for scattered rays (including a partial last wave)."""
+    spec = small_specs()[name]
+    prod, orc = both_arrays(spec)
+    ds = zo.DeviceScene(device, prod)
+    for label, rays in (("tiles", tiled_camera_rays(spec, 128, 72)),
+                        ("scattered", ray_set(spec, prod, 20001, seed=9))):
+        print(f"[trace] {name} packets, {label}: {len(rays)} rays", flush=True)
+        g = ds.trace(rays, spec.ray_cast_epsilon, packet=True)
+        r = ds.trace(rays, spec.ray_cast_epsilon)
+        o = O.OracleScene(orc).trace(rays, spec.ray_cast_epsilon)
+        assert np.array_equal(r["hit"], o["hit"])
+        assert g.tobytes() == r.tobytes(), label
+        assert o["hit"].mean() > 0.1
+
+
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
 def test_trace_rays_match_fp64_brute_force(device, name):
     """IDs equal on every ray the FP64 oracle does not flag (edge, tangency,

@@ -403,11 +403,13 @@ class DeviceScene:
     def set_trace_width(self, width):
         check(lib().orn_scene_set_trace_width(self._h, int(width)), "orn_scene_set_trace_width")
 
-    def trace(self, rays, t_min=0.001):
+    def trace(self, rays, t_min=0.001, packet=False):
+        """Closest hits (orn_trace_rays; with packet, orn_trace_rays_packet)."""
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
         out = np.zeros(len(r), dtype=HIT_DTYPE)
-        check(lib().orn_trace_rays(self._h, float(t_min), _ptr(r), len(r), _ptr(out)),
-              "orn_trace_rays")
+        fn = lib().orn_trace_rays_packet if packet else lib().orn_trace_rays
+        check(fn(self._h, float(t_min), _ptr(r), len(r), _ptr(out)),
+              "orn_trace_rays_packet" if packet else "orn_trace_rays")
         return out
 
     def destroy(self):

@@ -132,6 +132,7 @@ DEVICE_API = [
     ("orn_get_rng_state", st, [vp, vp, sz]),
     ("orn_set_state", st, [vp, vp, vp, sz]),
     ("orn_trace_rays", st, [vp, f32, vp, sz, vp]),
+    ("orn_trace_rays_packet", st, [vp, f32, vp, sz, vp]),
     ("orn_eval_math", st, [vp, C.c_int, vp, vp, vp, sz]),
     ("orn_calibration_copy", st, [vp, sz, fp]),
     ("orn_batch_create", st, [vp, P(BatchItem), sz, P(vp)]),

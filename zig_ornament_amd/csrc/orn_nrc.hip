@@ -764,7 +764,8 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count)
 // scattering surface becomes a cache query (the sample's value is then
 // throughput x prediction, added by the inference kernel); paths that end
 // earlier (sky, emitter, absorption, depth) keep their exact value.
-template <uint32_t F, bool PRIMARY>
+// (k = 0 has its own kernel, nrc_primary_kernel.)
+template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
                                                            uint32_t *__restrict__ rng_buf,
                                                            float4 *__restrict__ partial,
@@ -787,31 +788,7 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
         f3 thr = mk(1.0f, 1.0f, 1.0f);
         f3 color = thr;
         bool done = false;
-        if constexpr (PRIMARY) {  // query at the primary hit: no scattering at all
-            Hit h{};
-            done = true;
-            if (p.depth == 0u) {
-                done = false;
-            } else if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
-                color = thr * sky(ray.d);
-            } else {
-                const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
-                if (s.m.type <= 2u) {
-                    fill_record<F>(sc, s, ray, rc);
-                    rc.pixel = pix;
-                    rc.weight[0] = 1.0f;
-                    rc.weight[1] = 1.0f;
-                    rc.weight[2] = 1.0f;
-                    query = true;
-                    color = mk(0.0f, 0.0f, 0.0f);
-                } else {  // material_emit (material.wgsl:76-89)
-                    color = s.m.type == 3u ? (s.textured ? sample_texture(sc, s.m.texture, s.tu, s.tv) : s.m.albedo)
-                                           : mk(0.0f, 0.0f, 0.0f);
-                    color = thr * color;
-                }
-            }
-        }
-        for (uint32_t b = 0; !PRIMARY && b < p.depth && !done; b++) {
+        for (uint32_t b = 0; b < p.depth && !done; b++) {
             Hit h{};
             if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
                 color = thr * sky(ray.d);
@@ -841,6 +818,80 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
             }
         }
         if (!done) color = thr;
+        partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+        rng_buf[pix] = rng;
+    }
+    const uint32_t slot = wave_append(query, query_count);
+    if (query) queries[slot] = rc;
+}
+
+// The k = 0 case: the query is the primary hit, so each camera ray is traced
+// once and nothing scatters. A workgroup covers a 16 x 16 tile. In scenes
+// without meshes each wave takes an 8 x 8 quarter of it and traces its 64
+// coherent camera rays as one packet (bvh_hit_packet: every ray's own
+// closest hit, bit for bit; 1.6x faster than one traversal per lane on the
+// random-spheres scene). Finely tessellated meshes send the rays of a tile
+// down different leaves, where a packet walks the union of their paths
+// (6x slower on the instanced-icosphere scene), so there each lane traverses
+// alone (LDS stack) and a wave is a 16 x 4 strip.
+template <uint32_t F>
+__global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_constant_params p,
+                                                             uint32_t *__restrict__ rng_buf,
+                                                             float4 *__restrict__ partial,
+                                                             NrcRecord *__restrict__ queries,
+                                                             uint32_t *__restrict__ query_count)
+{
+    constexpr bool kPacket = !(F & kHasMeshes);
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t tiles_x = (p.width + 15u) / 16u;
+    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    const uint32_t x = (blockIdx.x % tiles_x) * 16u +
+                       (kPacket ? (wave % 2u) * 8u + lane % 8u : threadIdx.x % 16u);
+    const uint32_t y = (blockIdx.x / tiles_x) * 16u +
+                       (kPacket ? (wave / 2u) * 8u + lane / 8u : threadIdx.x / 16u);
+    const bool active = x < p.width && y < p.height;
+    const uint32_t pix = y * p.width + x;
+    uint32_t rng = 0u;
+    Ray ray;
+    ray.o = mk(0.0f, 0.0f, 0.0f);
+    ray.d = mk(1.0f, 1.0f, 1.0f);
+    if (active) {
+        rng = rng_buf[pix];
+        ray = camera_ray(p, x, y, rng);
+    }
+    Hit h{};
+    bool hit = false;
+    if constexpr (kPacket) {
+        hit = bvh_hit_packet<F>(sc, p.ray_cast_epsilon, ray, active && p.depth > 0u, sc.stack_slots, h);
+    } else if (active && p.depth > 0u) {
+        hit = bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, lds_stack + threadIdx.x, kBlock, h);
+    }
+    bool query = false;
+    NrcRecord rc;
+    if (active) {
+        const f3 thr = mk(1.0f, 1.0f, 1.0f);
+        f3 color = thr;  // depth 0: the loop never runs and the throughput is returned
+        if (p.depth > 0u) {
+            if (!hit) {
+                color = thr * sky(ray.d);
+            } else {
+                const Surface srf = surface<F>(sc, p.textures_count, 0u, ray, h);
+                if (srf.m.type <= 2u) {
+                    fill_record<F>(sc, srf, ray, rc);
+                    rc.pixel = pix;
+                    rc.weight[0] = 1.0f;
+                    rc.weight[1] = 1.0f;
+                    rc.weight[2] = 1.0f;
+                    query = true;
+                    color = mk(0.0f, 0.0f, 0.0f);
+                } else {  // material_emit (material.wgsl:76-89)
+                    color = srf.m.type == 3u
+                                ? (srf.textured ? sample_texture(sc, srf.m.texture, srf.tu, srf.tv) : srf.m.albedo)
+                                : mk(0.0f, 0.0f, 0.0f);
+                    color = thr * color;
+                }
+            }
+        }
         partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
         rng_buf[pix] = rng;
     }
@@ -1056,10 +1107,11 @@ void launch_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, 
 {
     const uint32_t tiles = blocks_for(p.width, 16u) * blocks_for(p.height, 16u);
     if (n->cfg.terminate_bounce == 0u)
-        nrc_paths_kernel<F, true><<<tiles, kBlock, lds, st>>>(ds, p, 0u, rng, n->partial, n->queries, n->counters);
+        nrc_primary_kernel<F><<<tiles, kBlock, (F & kHasMeshes) ? lds : 0, st>>>(ds, p, rng, n->partial, n->queries,
+                                                                                n->counters);
     else
-        nrc_paths_kernel<F, false><<<tiles, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, n->partial,
-                                                               n->queries, n->counters);
+        nrc_paths_kernel<F><<<tiles, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, n->partial, n->queries,
+                                                        n->counters);
 }
 
 template <uint32_t F>

@@ -473,9 +473,45 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
         o.material_index = h.material;
         o.node_type = h.node_type;
         o.inverted_transform_id = h.inv_tid;
-        o.tri_id = h.tri_id;
-        o.uv[0] = h.u;
-        o.uv[1] = h.v;
+        if (h.node_type == 2u) {  // triangle id and barycentrics exist for mesh hits only
+            o.tri_id = h.tri_id;
+            o.uv[0] = h.u;
+            o.uv[1] = h.v;
+        }
+    }
+    out[i] = o;
+}
+
+// The same query with the rays of a wave traced together (bvh_hit_packet);
+// every lane takes part, lanes past n without a ray.
+__global__ __launch_bounds__(kBlock) void trace_packet_kernel(DevScene sc, float t_min,
+                                                              const float *__restrict__ rays, uint32_t n,
+                                                              orn_hit *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    Ray r;
+    r.o = mk(0.0f, 0.0f, 0.0f);
+    r.d = mk(1.0f, 1.0f, 1.0f);
+    if (valid) {
+        r.o = mk(rays[6u * i], rays[6u * i + 1u], rays[6u * i + 2u]);
+        r.d = mk(rays[6u * i + 3u], rays[6u * i + 4u], rays[6u * i + 5u]);
+    }
+    Hit h{};
+    const bool hit = bvh_hit_packet<kHasSpheres | kHasMeshes>(sc, t_min, r, valid, sc.stack_slots, h);
+    if (!valid) return;
+    orn_hit o{};
+    o.hit = hit ? 1u : 0u;
+    if (hit) {
+        o.t = h.t;
+        o.material_index = h.material;
+        o.node_type = h.node_type;
+        o.inverted_transform_id = h.inv_tid;
+        if (h.node_type == 2u) {  // triangle id and barycentrics exist for mesh hits only
+            o.tri_id = h.tri_id;
+            o.uv[0] = h.u;
+            o.uv[1] = h.v;
+        }
     }
     out[i] = o;
 }
@@ -1293,7 +1329,8 @@ orn_status orn_set_state(orn_target *t, const float *accumulation, const uint32_
     return ORN_OK;
 }
 
-orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, size_t n, orn_hit *out)
+static orn_status trace_rays(orn_device_scene *s, float t_min, const float *rays, size_t n, orn_hit *out,
+                             bool packet)
 {
     if (!s) return ORN_ERROR_INVALID_HANDLE;
     if ((!rays || !out) && n) return ORN_ERROR_INVALID_VALUE;
@@ -1311,10 +1348,14 @@ orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, s
         ds.stack_slots = s->trace_width == 4u ? s->stack_slots4 : s->stack_slots;
         const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
         const uint32_t blocks = blocks_for(static_cast<uint32_t>(n), kBlock);
-        if (s->trace_width == 4u)
+        if (packet) {
+            ds.stack_slots = s->stack_slots;
+            trace_packet_kernel<<<blocks, kBlock, 0, st>>>(ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
+        } else if (s->trace_width == 4u) {
             trace_kernel<4><<<blocks, kBlock, lds, st>>>(ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
-        else
+        } else {
             trace_kernel<2><<<blocks, kBlock, lds, st>>>(ds, t_min, d_rays, static_cast<uint32_t>(n), d_out);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(orn_hit), hipMemcpyDeviceToHost, st);
@@ -1323,6 +1364,16 @@ orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, s
     if (d_rays) (void)hipFree(d_rays);
     if (d_out) (void)hipFree(d_out);
     return from_hip(e);
+}
+
+orn_status orn_trace_rays(orn_device_scene *s, float t_min, const float *rays, size_t n, orn_hit *out)
+{
+    return trace_rays(s, t_min, rays, n, out, false);
+}
+
+orn_status orn_trace_rays_packet(orn_device_scene *s, float t_min, const float *rays, size_t n, orn_hit *out)
+{
+    return trace_rays(s, t_min, rays, n, out, true);
 }
 
 orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, float *out, size_t n)

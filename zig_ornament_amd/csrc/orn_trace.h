@@ -379,6 +379,135 @@ __device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const R
     return tr.hit;
 }
 
+// bvh_hit for the 64 rays of a wave at once (BVH2). The wave walks the tree
+// one node at a time on one stack (entry i held by lane i of three registers:
+// the node and the 64-bit mask of the lanes that entered it). At an internal
+// node each active lane tests both child boxes with its own ray and t_max;
+// the wave goes on with the right child for the lanes whose ray hit it and
+// pushes the left child for the lanes whose ray hit that. So a lane is active
+// in exactly the nodes its own bvh_hit visits, meets them in the same order
+// with the same t_max, and ends with bvh_hit's result bit for bit. For
+// coherent rays (a camera tile) the wave visits few more nodes than one ray
+// does, with every lane busy and every node load at one address. Every lane
+// of the wave calls it (valid = false for a lane without a ray); `cap` is the
+// scene's BVH2 stack bound (at most 64).
+template <uint32_t F>
+__device__ __forceinline__ bool bvh_hit_packet(const DevScene &sc, float t_min, const Ray &world, bool valid,
+                                               uint32_t cap, Hit &h)
+{
+    Traversal tr;
+    trav_begin(sc.tlas2_root, world, tr);
+    const uint32_t lane = __lane_id();
+    uint64_t active = __ballot(valid);
+    uint32_t addr = sc.tlas2_root;
+    bool in_tlas = true;
+    uint32_t top = 0;
+    uint32_t s_node = 0, s_lo = 0, s_hi = 0;  // the wave's stack, entry i in lane i
+    auto push = [&](uint32_t node, uint64_t mask) {
+        if (top >= cap) return;
+        const bool mine = lane == top;  // lane `top` holds the new entry
+        s_node = mine ? node : s_node;
+        s_lo = mine ? static_cast<uint32_t>(mask) : s_lo;
+        s_hi = mine ? static_cast<uint32_t>(mask >> 32) : s_hi;
+        top++;
+    };
+    while (active) {
+        const bool me = (active >> lane) & 1ull;
+        if (!(addr & kLeafRef)) {
+            const float4 *node = (((F & kHasMeshes) && !in_tlas) ? sc.blas2 : sc.tlas2) + 4u * addr;
+            const float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
+            const bool l = me && aabb_hit(make_float3(n0.x, n0.y, n0.z), make_float3(n1.x, n1.y, n1.z), tr.inv,
+                                          tr.ox, t_min, tr.t_max);
+            const bool r = me && aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z), tr.inv,
+                                          tr.ox, t_min, tr.t_max);
+            const uint64_t ml = __ballot(l), mr = __ballot(r);
+            const uint32_t left = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(as_u32(n0.w))));
+            const uint32_t right = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(as_u32(n1.w))));
+            if (mr) {
+                if (ml) push(left, ml);
+                addr = right;
+                active = mr;
+                continue;
+            }
+            if (ml) {
+                addr = left;
+                active = ml;
+                continue;
+            }
+        } else {
+            const uint32_t idx = addr & ~kLeafRef;
+            const float4 *node = (((F & kHasMeshes) && !in_tlas) ? sc.blas2 : sc.tlas2) + 4u * idx;
+            const float4 n0 = node[0], n1 = node[1], n2 = node[2], n3 = node[3];
+            const uint32_t type = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(as_u32(n2.w))));
+            if ((F & kHasSpheres) && type == 1u) {
+                if (me) {
+                    const uint32_t tid = as_u32(n3.w) * 2u;
+                    const float t = sphere_hit(xform_ray(sc.transforms + 4u * tid, tr.ray), t_min, tr.t_max);
+                    if (t < tr.t_max) {
+                        tr.hit = true;
+                        tr.t_max = t;
+                        tr.h.t = t;
+                        tr.h.material = as_u32(n1.w);
+                        tr.h.node_type = 1u;
+                        tr.h.inv_tid = tid;
+                    }
+                }
+            } else if ((F & kHasMeshes) && type == 2u) {
+                if (me) {
+                    tr.inv_tid = as_u32(n3.w) * 2u;
+                    tr.material_index = as_u32(n1.w);
+                    tr.ray = xform_ray(sc.transforms + 4u * tr.inv_tid, tr.ray);
+                    tr.inv = safe_invdir(tr.ray.d);
+                    tr.ox = (-tr.ray.o) * tr.inv;
+                }
+                push(kFinishedTraverseBlas, 0ull);
+                in_tlas = false;
+                addr = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(as_u32(n0.w))));
+                continue;  // the same lanes go on into the BLAS
+            } else if ((F & kHasMeshes) && type == 3u) {
+                if (me) {
+                    float u = 0.0f, v = 0.0f;
+                    const float t = triangle_hit(tr.ray, xyz(n0), xyz(n1), xyz(n2), t_min, tr.t_max, u, v);
+                    if (t < tr.t_max) {
+                        tr.hit = true;
+                        tr.t_max = t;
+                        tr.h.t = t;
+                        tr.h.material = tr.material_index;
+                        tr.h.node_type = 2u;
+                        tr.h.inv_tid = tr.inv_tid;
+                        tr.h.tri_id = as_u32(n0.w) * 3u;
+                        tr.h.u = u;
+                        tr.h.v = v;
+                    }
+                }
+            }
+        }
+        // next pending entry; leaving a BLAS restores every lane's world ray
+        // (a no-op for the lanes that did not enter it)
+        active = 0ull;
+        while (top) {
+            --top;
+            const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s_node),
+                                                                               static_cast<int>(top)));
+            if ((F & kHasMeshes) && a == kFinishedTraverseBlas) {
+                in_tlas = true;
+                tr.ray = world;
+                tr.inv = safe_invdir(world.d);
+                tr.ox = (-world.o) * tr.inv;
+                continue;
+            }
+            addr = a;
+            active = (uint64_t(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s_hi),
+                                                                               static_cast<int>(top))))
+                      << 32) |
+                     static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s_lo), static_cast<int>(top)));
+            break;
+        }
+    }
+    h = tr.h;
+    return tr.hit;
+}
+
 
 // ---------------------------------------------------------------------------
 // Shading: material.wgsl, hitrecord.wgsl, pathtracer.wgsl:95-161
