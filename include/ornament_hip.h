@@ -335,7 +335,7 @@ orn_status orn_batch_autotune(orn_batch *batch, const orn_constant_params *param
 
 /* ------------------------------------------------------------------ */
 /* Neural radiance cache (no counterpart in the reference: an addition  */
-/* that changes the method; the plain tracer above stays the judge)     */
+/* that changes the method; the plain tracer above stays the reference) */
 /* ------------------------------------------------------------------ */
 
 /* A multiresolution hash grid (16 levels x 2 FP16 features, 2^log2_table
