@@ -385,6 +385,20 @@ def build_scene(spec):
     return s
 
 
+def spec_params(spec, current_iteration=0.0):
+    """The ConstantParams (gpu_structs.zig:61-86) of a spec's render."""
+    from . import _native
+    p = _native.ConstantParams()
+    p.camera = build_scene_camera(spec)
+    p.depth, p.width, p.height = spec.depth, spec.width, spec.height
+    p.flip_y = 1 if spec.flip_y else 0
+    p.inverted_gamma = 1.0 / spec.gamma
+    p.ray_cast_epsilon = spec.ray_cast_epsilon
+    p.textures_count = len(spec.textures)
+    p.current_iteration = current_iteration
+    return p
+
+
 def spec_settings(spec):
     """The render settings of a spec, as stored in a scene file."""
     return dict(width=spec.width, height=spec.height, iterations=spec.spp, depth=spec.depth,
