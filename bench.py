@@ -5,6 +5,9 @@ r=0.2 spheres + three r=1 spheres), 1920x1080, 64 samples per pixel, depth 16,
 seed 0. One step = one PathTracer.render() of 64 spp over the whole image
 (wgpu_backend/path_tracer.zig:252-265), i.e. 132,710,400 samples, with the
 scene already resident in HBM. Metric: samples/s = W*H*spp*steps / wall time.
+`--workload C1` (Cornell box 256x256, 16 spp, depth 8) or `--workload C3` (1,024
+instanced icospheres, 1920x1080, 32 spp, depth 8) times those scenes the same
+way (the profiled HBM traffic is reported for C2 only).
 
 Launch tuning: the library's autotuner picks the render kernel's launch
 parameters for the scene class; its JSON cache ships with the package
@@ -53,6 +56,12 @@ import zig_ornament_amd as zo  # noqa: E402
 from zig_ornament_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# --workload: (config name, metric text); C2 is the headline configuration
+WORKLOADS = {
+    "C1": ("cornell_256x256_16spp_depth8", "Cornell box 256x256 16spp depth 8"),
+    "C2": ("random_spheres_1920x1080_64spp_depth16", "random-spheres 1920x1080 64spp depth 16"),
+    "C3": ("instanced_icospheres_1920x1080_32spp_depth8", "1,024 instanced icospheres 1920x1080 32spp depth 8"),
+}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "render_kernel_random_spheres_1080p_64spp_pmc.md")
 
 
@@ -242,6 +251,9 @@ def main():
                     help="autotuner cache (JSON); a scene class missing from it is measured")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch tuning")
     ap.add_argument("--no-nrc", action="store_true", help="skip the equal-time RMSE leg")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="C2",
+                    help="the timed scene (C2, random spheres, is the headline; C1 and C3 print the "
+                         "same line for the Cornell box and the instanced icospheres)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path tracer shards by pixel tiles with no "
@@ -249,7 +261,8 @@ def main():
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    spec = W.random_spheres()  # 1920x1080, 64 spp, depth 16
+    spec = W.CONFIGS[args.workload]()
+    workload, description = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     scene = W.build_scene(spec)
     arrays = zo.Bvh(scene)
@@ -292,7 +305,7 @@ def main():
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
 
     out = {
-        "metric": "samples/s (plain path tracing, random-spheres 1920x1080 64spp depth 16); "
+        "metric": f"samples/s (plain path tracing, {description}); "
                   "equal-time RMSE vs 4096-spp reference, plain PT vs NRC",
         "value": value,
         "unit": "samples/s",
@@ -305,7 +318,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 scene generator)",
-        "config": {"workload": "random_spheres_1920x1080_64spp_depth16", "width": spec.width,
+        "config": {"workload": workload, "width": spec.width,
                    "height": spec.height, "spp": spec.spp, "depth": spec.depth,
                    "spheres": len(spec.spheres), "tlas_nodes": int(len(arrays.tlas_nodes)),
                    "parallelism": "single GPU, pixel-parallel", "device": name,
@@ -321,7 +334,7 @@ def main():
         achieved = bytes_per_launch / avg_kernel_s / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": profiled_traffic(),
+                           "traffic": profiled_traffic() if args.workload == "C2" else None,
                            "algorithmic_bytes_per_sample": cb["bytes_per_sample"],
                            "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
                            "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"],
@@ -329,7 +342,7 @@ def main():
         out["cpu_baseline"] = {"value": cb["samples_per_s"], "unit": "samples/s",
                                "cores": cb["threads"], "kind": "port",
                                "sample": f"{cb['pixels']} evenly spaced pixels of the same image "
-                                         f"at 64 spp ({cb['pixels'] * spec.spp} samples, "
+                                         f"at {spec.spp} spp ({cb['pixels'] * spec.spp} samples, "
                                          f"{cb['seconds']:.1f} s) on {cb['threads']} host threads"}
         out["gpu_over_cpu"] = value / cb["samples_per_s"]
     if not args.no_nrc:
