@@ -207,8 +207,8 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
     slowest = max(r["ms_per_frame"] for r in runs.values())
     plain = []
     pt = zo.Target(ds, spec.width, spec.height, 7)
-    pt.set_tuning(**ref.tuning())
     for spp in (1, 2, 3, 4, 6, 8, 12, 16, 24, 32):
+        pt.autotune(p, spp, cache_path=zo.TUNING_CACHE)  # the plain tracer's best launch for this spp
         pt.set_state(np.zeros((n, 4), np.float32), np.arange(n, dtype=np.uint32) + np.uint32(7 * n))
         pt.render(p, spp)
         img = pt.accumulation() / spp

@@ -121,7 +121,9 @@ def test_trace_rays_match_fp64_brute_force(device, name):
                                     dict(bvh_width=2, regroup_lanes=64, workgroups_per_cu=3),
                                     dict(samples_per_job=1, split_percent=100, regroup_lanes=8),
                                     dict(samples_per_job=0),
-                                    dict(samples_per_job=3, split_percent=60, workgroups_per_cu=1)])
+                                    dict(samples_per_job=3, split_percent=60, workgroups_per_cu=1),
+                                    dict(regroup_lanes=32, leaf_regroup_lanes=8, samples_per_job=4,
+                                         split_percent=100)])
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
 def test_image_matches_oracle(device, name, tuning):
     """Any launch tuning (node width, regrouping, persistent grid size, job

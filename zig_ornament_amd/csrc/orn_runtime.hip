@@ -2091,7 +2091,7 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
     static const uint32_t per_cu[] = {0, 1, 2, 3};
     static const uint32_t width[] = {2, 4};
     static const uint32_t split[] = {25, 50, 100};
-    static const uint32_t job[] = {4, 8, 16, 32};
+    static const uint32_t job[] = {2, 4, 8, 16, 32};
     auto sweep = [&](uint32_t orn_tuning::*field, const uint32_t *vals, size_t n) -> orn_status {
         for (size_t i = 0; i < n; i++) {
             orn_tuning t = cur;
@@ -2106,24 +2106,31 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
         }
         return ORN_OK;
     };
-    if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
-    if ((st = sweep(&orn_tuning::leaf_regroup_lanes, leaf, 5)) != ORN_OK) return st;
-    if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
-    if ((st = sweep(&orn_tuning::workgroups_per_cu, per_cu, 4)) != ORN_OK) return st;
-    // job splitting: both fields at once (they only pay off together)
-    for (uint32_t sp : split)
-        for (uint32_t jb : job) {
-            orn_tuning t = cur;
-            t.split_percent = sp;
-            t.samples_per_job = jb;
-            float ms = 0;
-            if ((st = measure(t, &ms)) != ORN_OK) return st;
-            if (ms < cur_ms) {
-                cur = t;
-                cur_ms = ms;
+    // Passes over every parameter until one changes nothing (at most 4): the
+    // parameters interact (small jobs pay off only with wider shading rounds,
+    // for instance), so one pass can stop far from the best setting.
+    for (int pass = 0; pass < 4; pass++) {
+        const float before = cur_ms;
+        if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
+        if ((st = sweep(&orn_tuning::leaf_regroup_lanes, leaf, 5)) != ORN_OK) return st;
+        if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
+        if ((st = sweep(&orn_tuning::workgroups_per_cu, per_cu, 4)) != ORN_OK) return st;
+        // job splitting: both fields at once (they only pay off together)
+        for (uint32_t sp : split)
+            for (uint32_t jb : job) {
+                orn_tuning t = cur;
+                t.split_percent = sp;
+                t.samples_per_job = jb;
+                float ms = 0;
+                if ((st = measure(t, &ms)) != ORN_OK) return st;
+                if (ms < cur_ms) {
+                    cur = t;
+                    cur_ms = ms;
+                }
             }
-        }
-    if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
+        if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
+        if (!(cur_ms < before)) break;
+    }
     *best = cur;
     *best_ms = cur_ms;
     return ORN_OK;
