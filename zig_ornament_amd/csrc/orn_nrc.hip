@@ -826,14 +826,14 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
 }
 
 // The k = 0 case: the query is the primary hit, so each camera ray is traced
-// once and nothing scatters. A workgroup covers a 16 x 16 tile. In scenes
-// without meshes each wave takes an 8 x 8 quarter of it and traces its 64
+// once and nothing scatters. A workgroup covers a 16 x 16 tile, each wave an
+// 8 x 8 quarter of it. In scenes without meshes the wave traces its 64
 // coherent camera rays as one packet (bvh_hit_packet: every ray's own
 // closest hit, bit for bit; 1.6x faster than one traversal per lane on the
 // random-spheres scene). Finely tessellated meshes send the rays of a tile
 // down different leaves, where a packet walks the union of their paths
-// (6x slower on the instanced-icosphere scene), so there each lane traverses
-// alone (LDS stack) and a wave is a 16 x 4 strip.
+// (6x slower on the instanced-icosphere scene), so there each lane
+// traverses alone (LDS stack).
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_constant_params p,
                                                              uint32_t *__restrict__ rng_buf,
@@ -845,10 +845,8 @@ __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_co
     extern __shared__ uint32_t lds_stack[];
     const uint32_t tiles_x = (p.width + 15u) / 16u;
     const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
-    const uint32_t x = (blockIdx.x % tiles_x) * 16u +
-                       (kPacket ? (wave % 2u) * 8u + lane % 8u : threadIdx.x % 16u);
-    const uint32_t y = (blockIdx.x / tiles_x) * 16u +
-                       (kPacket ? (wave / 2u) * 8u + lane / 8u : threadIdx.x / 16u);
+    const uint32_t x = (blockIdx.x % tiles_x) * 16u + (wave % 2u) * 8u + lane % 8u;
+    const uint32_t y = (blockIdx.x / tiles_x) * 16u + (wave / 2u) * 8u + lane / 8u;
     const bool active = x < p.width && y < p.height;
     const uint32_t pix = y * p.width + x;
     uint32_t rng = 0u;
