@@ -37,8 +37,8 @@ Also reported:
                the plain RMSE in the cache's frame time, and the plain time
                needed to reach the cache's RMSE. Two training schedules:
                `nrc` as the workload specifies it (2^19-entry tables, 65,536
-               training paths and four Adam steps of 16,384 records per
-               frame) and `nrc_light` (16,384 paths, one step).
+               training paths per frame, one Adam step over their records)
+               and `nrc_light` (16,384 paths, one step).
 """
 import argparse
 import json
@@ -146,11 +146,11 @@ def relmse(img, ref):
 
 
 # The radiance-cache runs of the equal-time comparison: the cache as the
-# workload names it (2^19-entry tables, 65,536 training paths and four Adam
-# steps of 16,384 records per frame), and a lighter schedule (16,384 paths,
-# one step) that spends less of each frame on training.
+# workload names it (2^19-entry tables, 65,536 training paths per frame,
+# their records taken by one Adam step), and a lighter schedule (16,384
+# paths, one step) that spends less of each frame on training.
 NRC_RUNS = (
-    ("nrc", dict(log2_table=19, train_paths=65536, train_batch=16384), 4),
+    ("nrc", dict(log2_table=19, train_paths=65536, train_batch=65536), 1),
     ("nrc_light", dict(log2_table=19, train_paths=16384, train_batch=16384), 1),
 )
 

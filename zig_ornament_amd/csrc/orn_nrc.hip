@@ -490,29 +490,36 @@ enum GradMode : int {
 };
 
 // MLP parameters: dW = sum over the training workgroups' partials
-// (partials[group][j], summed in a fixed order: groups slice, slice + 4, ...
-// per wave, then the four wave sums), then store it or take an Adam step,
-// and rewrite the transposed, padded bf16 copy the kernels read. One
-// workgroup covers 64 parameters; the parameter count is a multiple of 64.
+// (partials[group][j], summed in a fixed order: wave s of the workgroup adds
+// groups s, s + 16, ... with four loads in flight, then the 16 wave sums are
+// added in order), then store it or take an Adam step, and rewrite the
+// transposed, padded bf16 copy the kernels read. One workgroup covers 64
+// parameters; the parameter count is a multiple of 64.
+constexpr uint32_t kMlpSlices = 16;
 static_assert(kMlpParams % 64u == 0u, "MLP parameters per workgroup");
-__global__ __launch_bounds__(kBlock) void nrc_mlp_kernel(const float *__restrict__ partials, uint32_t groups,
-                                                         AdamStep a, float *__restrict__ w, float *__restrict__ grad,
-                                                         float *__restrict__ m, float *__restrict__ v,
-                                                         uint16_t *__restrict__ wt, int mode)
+__global__ __launch_bounds__(64u * kMlpSlices) void nrc_mlp_kernel(const float *__restrict__ partials, uint32_t groups,
+                                                                   AdamStep a, float *__restrict__ w,
+                                                                   float *__restrict__ grad, float *__restrict__ m,
+                                                                   float *__restrict__ v, uint16_t *__restrict__ wt,
+                                                                   int mode)
 {
-    __shared__ float s_sum[kBlock / 64u][64];
+    __shared__ float s_sum[kMlpSlices][64];
     const uint32_t lane = threadIdx.x % 64u, slice = threadIdx.x / 64u;
     const uint32_t j = blockIdx.x * 64u + lane;
     if (mode != kRefresh) {
-        float acc = 0.0f;
-        for (uint32_t gi = slice; gi < groups; gi += kBlock / 64u) acc = acc + partials[size_t(gi) * kMlpParams + j];
-        s_sum[slice][lane] = acc;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t gi = slice;
+        for (; gi + 3u * kMlpSlices < groups; gi += 4u * kMlpSlices)
+            for (uint32_t u = 0; u < 4u; u++) acc[u] = acc[u] + partials[size_t(gi + u * kMlpSlices) * kMlpParams + j];
+        for (; gi < groups; gi += kMlpSlices) acc[0] = acc[0] + partials[size_t(gi) * kMlpParams + j];
+        s_sum[slice][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
     __syncthreads();
     if (slice != 0u) return;
     float p = w[j];
     if (mode != kRefresh) {
-        const float gj = (s_sum[0][lane] + s_sum[1][lane]) + (s_sum[2][lane] + s_sum[3][lane]);
+        float gj = 0.0f;
+        for (uint32_t k = 0; k < kMlpSlices; k++) gj = gj + s_sum[k][lane];
         grad[j] = gj;
         if (mode == kUpdate) {
             float mj = m[j], vj = v[j];
@@ -1175,8 +1182,8 @@ AdamStep adam_step(const orn_nrc *n)
 hipError_t refresh_params(orn_nrc *n, hipStream_t st)
 {
     const AdamStep a = adam_step(n);
-    nrc_mlp_kernel<<<kMlpParams / 64u, kBlock, 0, st>>>(nullptr, 0u, a, n->mlp, n->mlp_grad, n->mlp_m, n->mlp_v,
-                                                       n->mlp_wt, kRefresh);
+    nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(nullptr, 0u, a, n->mlp, n->mlp_grad, n->mlp_m,
+                                                                 n->mlp_v, n->mlp_wt, kRefresh);
     const uint64_t g = n->grid_params;
     nrc_grid_half_kernel<<<static_cast<uint32_t>((g + 255) / 256), 256, 0, st>>>(n->grid_w, n->grid_half, g);
     return hipGetLastError();
@@ -1194,8 +1201,8 @@ hipError_t gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr
                                                 buf, count_ptr, count_max, first, batch, n->partials, n->sample_u,
                                                 n->sample_dx, n->train_cap, n->loss);
     const AdamStep a = adam_step(n);
-    nrc_mlp_kernel<<<kMlpParams / 64u, kBlock, 0, st>>>(n->partials, groups, a, n->mlp, n->mlp_grad, n->mlp_m,
-                                                       n->mlp_v, n->mlp_wt, mode);
+    nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(n->partials, groups, a, n->mlp, n->mlp_grad,
+                                                                 n->mlp_m, n->mlp_v, n->mlp_wt, mode);
     GridUpdate up;
     up.a = a;
     up.w = n->grid_w;
