@@ -238,7 +238,12 @@ orn_status orn_target_wait(orn_target *target);
  *   consecutive samples, handed from lane to lane through memory, so that no
  *   lane is left with a whole pixel's samples at the end of a render (default
  *   0 and 16; either 0: one job per pixel). Renders whose current_iteration
- *   is not a whole number below 2^24 - iterations run one job per pixel. */
+ *   is not a whole number below 2^24 - iterations run one job per pixel.
+ * node_steps (0..16): internal BVH nodes a traversing lane visits in a row
+ *   before its wave looks for leaf and shading rounds again; fewer checks
+ *   cost fewer instructions, while lanes that reach a leaf or finish their
+ *   ray early wait longer (0 takes the default, 4).
+ * reserved: must be 0. */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
@@ -246,7 +251,8 @@ typedef struct orn_tuning {
     uint32_t leaf_regroup_lanes;
     uint32_t samples_per_job;
     uint32_t split_percent;
-    uint32_t reserved[2];
+    uint32_t node_steps;
+    uint32_t reserved;
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);

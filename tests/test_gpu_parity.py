@@ -123,11 +123,14 @@ def test_trace_rays_match_fp64_brute_force(device, name):
                                     dict(samples_per_job=0),
                                     dict(samples_per_job=3, split_percent=60, workgroups_per_cu=1),
                                     dict(regroup_lanes=32, leaf_regroup_lanes=8, samples_per_job=4,
-                                         split_percent=100)])
+                                         split_percent=100),
+                                    dict(node_steps=1), dict(node_steps=16, leaf_regroup_lanes=1),
+                                    dict(node_steps=6, regroup_lanes=8, samples_per_job=2, split_percent=100)])
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
 def test_image_matches_oracle(device, name, tuning):
     """Any launch tuning (node width, regrouping, persistent grid size, job
-    size) renders the oracle's image bit for bit."""
+    size, internal-node steps between round checks) renders the oracle's
+    image bit for bit."""
     spec = small_specs()[name]
     prod, _ = both_arrays(spec)
     n = spec.width * spec.height

@@ -134,6 +134,31 @@ def test_device_errors(device):
     assert ds.trace(np.zeros((0, 6), np.float32)).shape == (0,)
 
 
+def test_tuning_validation_and_defaults(device):
+    """orn_target_set_tuning rejects out-of-range fields (node_steps above
+    16, a nonzero reserved word) and stores node_steps 0 as its default."""
+    import ctypes as C
+    from zig_ornament_amd import _native as N
+    spec = W.cornell(width=16, height=16, spp=1)
+    ds = zo.DeviceScene(device, zo.Bvh(W.build_scene(spec)))
+    t = zo.Target(ds, 16, 16)
+    assert t.tuning()["node_steps"] == 4
+    t.set_tuning(node_steps=0)
+    assert t.tuning()["node_steps"] == 4
+    t.set_tuning(node_steps=16, regroup_lanes=8)
+    assert t.tuning()["node_steps"] == 16 and t.tuning()["regroup_lanes"] == 8
+    for bad in (dict(node_steps=17), dict(regroup_lanes=0), dict(leaf_regroup_lanes=65),
+                dict(bvh_width=3), dict(split_percent=101)):
+        with pytest.raises(zo.OrnamentError) as e:
+            t.set_tuning(**bad)
+        assert e.value.status == 1, bad  # ORN_ERROR_INVALID_VALUE
+    raw = N.Tuning()
+    assert N.lib().orn_target_get_tuning(t._h, C.byref(raw)) == 0
+    raw.reserved = 1
+    assert N.lib().orn_target_set_tuning(t._h, C.byref(raw)) == 1  # ORN_ERROR_INVALID_VALUE
+    assert t.tuning()["node_steps"] == 16
+
+
 def test_fractional_iteration_counter_matches_oracle(device):
     """A current_iteration that is not a whole number (never produced by
     State, but accepted by the ABI) keeps the reference's per-sample f32

@@ -16,6 +16,7 @@ tuning) for comparison.
 """
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -68,6 +69,9 @@ def main(argv=None):
                         np.arange(s.width * s.height, dtype=np.uint32) + np.uint32(s.seed * s.width * s.height))
             t.render(p, s.spp)
             total += t.last_render_ms()
+            print(f"[sequential] image {len(out.setdefault('sequential_ms', [])) + 1} of {len(specs)}: "
+                  f"{t.last_render_ms():.3f} ms", file=sys.stderr, flush=True)
+            out["sequential_ms"].append(t.last_render_ms())
             t.destroy()
             ds.destroy()
         out["sequential_kernel_ms"] = total

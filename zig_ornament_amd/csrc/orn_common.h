@@ -48,6 +48,11 @@ ORN_HD f3 normalize(f3 a)
 }
 ORN_HD f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 
+// Mask of the wave's active lanes for which b holds. Taking the bool directly
+// lets a comparison feed the mask; HIP's __ballot(int) first widens it to a
+// lane value and compares that again.
+__device__ __forceinline__ uint64_t lanes_where(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
 ORN_HD uint32_t as_u32(float f) { return __builtin_bit_cast(uint32_t, f); }
 ORN_HD float as_f32(uint32_t u) { return __builtin_bit_cast(float, u); }
 

@@ -40,6 +40,8 @@ using namespace orn_rt;
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// internal BVH nodes a lane visits in a row before a leaf visit (bvh_hit)
+constexpr uint32_t kNodeSteps = 4;
 constexpr uint32_t kIn = 64;     // network input width
 constexpr uint32_t kW = 64;      // hidden width
 constexpr uint32_t kOut = 16;    // output width (RGB + padding)
@@ -740,7 +742,7 @@ __device__ __forceinline__ f3 continue_path(const DevScene &sc, const orn_consta
 {
     for (uint32_t b = b0; b < p.depth; b++) {
         Hit h{};
-        if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) return thr * sky(ray.d);
+        if (!bvh_hit<F, 2, kNodeSteps>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) return thr * sky(ray.d);
         Ray next;
         f3 w;
         if (shade<F>(sc, p.textures_count, 0u, ray, h, rng, next, w)) {
@@ -757,7 +759,7 @@ __device__ __forceinline__ f3 continue_path(const DevScene &sc, const orn_consta
 // one atomic per wave on the shared counter.
 __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count)
 {
-    const uint64_t m = __ballot(want);
+    const uint64_t m = lanes_where(want);
     if (m == 0u) return 0u;
     const uint32_t lane = threadIdx.x % 64u;
     const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
@@ -797,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
         bool done = false;
         for (uint32_t b = 0; b < p.depth && !done; b++) {
             Hit h{};
-            if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
+            if (!bvh_hit<F, 2, kNodeSteps>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) {
                 color = thr * sky(ray.d);
                 done = true;
                 break;
@@ -869,7 +871,7 @@ __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_co
     if constexpr (kPacket) {
         hit = bvh_hit_packet<F>(sc, p.ray_cast_epsilon, ray, active && p.depth > 0u, sc.stack_slots, h);
     } else if (active && p.depth > 0u) {
-        hit = bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, lds_stack + threadIdx.x, kBlock, h);
+        hit = bvh_hit<F, 2, kNodeSteps>(sc, p.ray_cast_epsilon, ray, lds_stack + threadIdx.x, kBlock, h);
     }
     bool query = false;
     NrcRecord rc;
@@ -941,7 +943,7 @@ __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, or
         Ray ray = camera_ray(p, pix % p.width, pix / p.width, rng);
         for (uint32_t b = 0; b <= k && b < p.depth; b++) {
             Hit h{};
-            if (!bvh_hit<F, 2>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) break;
+            if (!bvh_hit<F, 2, kNodeSteps>(sc, p.ray_cast_epsilon, ray, stack, kBlock, h)) break;
             const Surface s = surface<F>(sc, p.textures_count, 0u, ray, h);
             Ray next;
             f3 w;
@@ -965,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void nrc_train_paths_kernel(DevScene sc, or
         if (keep) stage[j] = rc;
     }
     __shared__ uint32_t s_kept[kBlock / 64u];
-    const uint64_t m = __ballot(keep);
+    const uint64_t m = lanes_where(keep);
     if (threadIdx.x % 64u == 0u) s_kept[threadIdx.x / 64u] = static_cast<uint32_t>(__popcll(m));
     __syncthreads();
     if (threadIdx.x == 0u) {
@@ -1015,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(const NrcRecord *__
     const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     const bool keep = j < n_paths && keep_flags[j] != 0u;
-    const uint64_t m = __ballot(keep);
+    const uint64_t m = lanes_where(keep);
     if (lane == 0u) s_wave[wave] = static_cast<uint32_t>(__popcll(m));
     __syncthreads();
     uint32_t in_block = 0;

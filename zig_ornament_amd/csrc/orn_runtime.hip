@@ -50,6 +50,7 @@ struct KernelParams {
     uint32_t tiles_per_xcd;
     uint32_t regroup;       // waiting lanes that trigger a shading round
     uint32_t leaf_regroup;  // lanes at a leaf that trigger a leaf round
+    uint32_t node_steps;    // internal nodes per lane between round checks
     uint32_t root;          // TLAS root reference (single image)
     uint32_t chunk;         // samples per job of a split pixel (single image)
     uint32_t chunks;        // jobs per split pixel (single image)
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     // have completed: a wait on all of the wave's memory operations, which by
     // the next shading round have long been done, so the wait is short.
     auto flush_releases = [&]() {
-        if (__ballot(release_tag != 0u) != 0u) {
+        if (lanes_where(release_tag != 0u) != 0u) {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             if (release_tag != 0u) {
                 store_coherent(stage + release_pix, release_tag);
@@ -325,21 +326,28 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     uint32_t band_jobs = band_px - band_split + band_split * max_chunks;
     const uint32_t regroup = kp.regroup;
     const uint32_t leaf_regroup = kp.leaf_regroup;
+    const uint32_t node_steps = kp.node_steps;
     for (;;) {
-        // internal nodes every iteration; leaves (sphere, triangle, mesh
-        // entry: the long code) in rounds of at least `leaf_regroup` lanes
-        if (mode == kTraverse && !(tr.addr & kLeafRef) &&
-            !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr))
-            mode = kWaiting;
-        const uint64_t at_leaf = __ballot(mode == kTraverse && (tr.addr & kLeafRef) != 0u);
-        const uint64_t at_inner = __ballot(mode == kTraverse && (tr.addr & kLeafRef) == 0u);
+        // up to `node_steps` internal nodes per lane every iteration; leaves
+        // (sphere, triangle, mesh entry: the long code) in rounds of at
+        // least `leaf_regroup` lanes
+        for (uint32_t k = 0; k < node_steps; k++) {
+            if (mode == kTraverse && !(tr.addr & kLeafRef) &&
+                !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr))
+                mode = kWaiting;
+        }
+        // masks of single comparisons, combined on the scalar unit
+        const uint64_t tracing = lanes_where(mode == kTraverse);
+        const uint64_t leaf_next = lanes_where((tr.addr & kLeafRef) != 0u);
+        const uint64_t at_leaf = tracing & leaf_next;
+        const uint64_t at_inner = tracing & ~leaf_next;
         if (at_leaf != 0u && (static_cast<uint32_t>(__popcll(at_leaf)) >= leaf_regroup || at_inner == 0u)) {
             if (mode == kTraverse && (tr.addr & kLeafRef) &&
                 !trav_leaf<F, W>(sc, t_min(), ray, stack, kBlock, tr))
                 mode = kWaiting;
         }
-        const uint64_t waiting = __ballot(mode == kWaiting);
-        const uint64_t traversing = __ballot(mode == kTraverse);
+        const uint64_t waiting = lanes_where(mode == kWaiting);
+        const uint64_t traversing = lanes_where(mode == kTraverse);
         if (waiting == 0u && traversing == 0u) break;
         if (static_cast<uint32_t>(__popcll(waiting)) < regroup && traversing != 0u) continue;
         // shading round
@@ -377,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         // jobs past an image's last) are skipped; a band that runs out
         // sends the wave on to the next one
         for (;;) {
-            const uint64_t needy = __ballot(mode == kWaiting && need_pixel);
+            const uint64_t needy = lanes_where(mode == kWaiting && need_pixel);
             if (needy == 0u) break;
             const int leader = __ffsll(static_cast<unsigned long long>(needy)) - 1;
             const uint32_t count = static_cast<uint32_t>(__popcll(needy));
@@ -775,6 +783,7 @@ orn_tuning default_tuning()
     t.leaf_regroup_lanes = 16;
     t.samples_per_job = 16;
     t.split_percent = 0;
+    t.node_steps = 4;
     return t;
 }
 
@@ -782,7 +791,15 @@ bool tuning_valid(const orn_tuning *t)
 {
     return t && t->regroup_lanes >= 1 && t->regroup_lanes <= 64 && t->workgroups_per_cu <= 64 &&
            (t->bvh_width == 2u || t->bvh_width == 4u) && t->leaf_regroup_lanes >= 1 &&
-           t->leaf_regroup_lanes <= 64 && t->samples_per_job <= 0xffffu && t->split_percent <= 100u;
+           t->leaf_regroup_lanes <= 64 && t->samples_per_job <= 0xffffu && t->split_percent <= 100u &&
+           t->node_steps <= 16u && t->reserved == 0u;
+}
+
+// node_steps 0 stands for the default
+orn_tuning normalized(orn_tuning t)
+{
+    if (t.node_steps == 0u) t.node_steps = default_tuning().node_steps;
+    return t;
 }
 
 // Jobs of one pixel's render: `chunk` samples each, `chunks` of them. A render
@@ -1220,6 +1237,7 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
     kp.regroup = t->tuning.regroup_lanes;
     kp.leaf_regroup = t->tuning.leaf_regroup_lanes;
+    kp.node_steps = t->tuning.node_steps;
     const uint32_t width = t->tuning.bvh_width == 2u ? 2u : 4u;
     kp.root = width == 4u ? s->ds.tlas4_root : s->ds.tlas2_root;
     plan_jobs(iterations, params->current_iteration, t->tuning.samples_per_job,
@@ -1265,7 +1283,7 @@ orn_status orn_target_set_tuning(orn_target *t, const orn_tuning *tuning)
 {
     if (!t) return ORN_ERROR_INVALID_HANDLE;
     if (!tuning_valid(tuning)) return ORN_ERROR_INVALID_VALUE;
-    t->tuning = *tuning;
+    t->tuning = normalized(*tuning);
     return ORN_OK;
 }
 
@@ -1636,6 +1654,7 @@ orn_status capture_batch(orn_batch *b)
     KernelParams kp{};
     kp.regroup = b->tuning.regroup_lanes;
     kp.leaf_regroup = b->tuning.leaf_regroup_lanes;
+    kp.node_steps = b->tuning.node_steps;
     kp.split_pct = b->tuning.split_percent;
     kp.pixels = b->total_px;
     LaunchArgs a{kp, b->d_items, b->d_work, b->acc, b->rng, b->fb, b->stage, b->queues, b->total_tiles};
@@ -1826,10 +1845,11 @@ orn_status orn_batch_set_tuning(orn_batch *b, const orn_tuning *tuning)
 {
     if (!b) return ORN_ERROR_INVALID_HANDLE;
     if (!tuning_valid(tuning)) return ORN_ERROR_INVALID_VALUE;
-    if (std::memcmp(tuning, &b->tuning, sizeof(orn_tuning)) != 0) {
+    const orn_tuning n = normalized(*tuning);
+    if (std::memcmp(&n, &b->tuning, sizeof(orn_tuning)) != 0) {
         ORN_TRY(hipSetDevice(b->scene->ctx->device));
         ORN_TRY(hipStreamSynchronize(b->scene->ctx->stream));  // the graph may be running
-        b->tuning = *tuning;
+        b->tuning = n;
         b->graph_ok = false;  // captured again at the next render
     }
     return ORN_OK;
@@ -2029,11 +2049,12 @@ bool cached_tuning(const char *path, const std::string &key, orn_tuning *t, floa
             else if (f.first == "bvh_width") r.bvh_width = v, ok[3] = true;
             else if (f.first == "samples_per_job") r.samples_per_job = v;
             else if (f.first == "split_percent") r.split_percent = v;
+            else if (f.first == "node_steps") r.node_steps = v;
             else if (f.first == "ms") *ms = static_cast<float>(f.second);
         }
         if (!(ok[0] && ok[1] && ok[2] && ok[3])) return false;
         if (!tuning_valid(&r)) return false;
-        *t = r;
+        *t = normalized(r);
         return true;
     }
     return false;
@@ -2050,6 +2071,7 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
         {"bvh_width", t.bvh_width},
         {"samples_per_job", t.samples_per_job},
         {"split_percent", t.split_percent},
+        {"node_steps", t.node_steps},
         {"ms", ms}};
     bool found = false;
     for (auto &e : c)
@@ -2092,6 +2114,7 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
     static const uint32_t width[] = {2, 4};
     static const uint32_t split[] = {25, 50, 100};
     static const uint32_t job[] = {2, 4, 8, 16, 32};
+    static const uint32_t steps[] = {1, 2, 4, 6, 8, 12};
     auto sweep = [&](uint32_t orn_tuning::*field, const uint32_t *vals, size_t n) -> orn_status {
         for (size_t i = 0; i < n; i++) {
             orn_tuning t = cur;
@@ -2113,6 +2136,7 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
         const float before = cur_ms;
         if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
         if ((st = sweep(&orn_tuning::leaf_regroup_lanes, leaf, 5)) != ORN_OK) return st;
+        if ((st = sweep(&orn_tuning::node_steps, steps, 6)) != ORN_OK) return st;
         if ((st = sweep(&orn_tuning::regroup_lanes, regroup, 6)) != ORN_OK) return st;
         if ((st = sweep(&orn_tuning::workgroups_per_cu, per_cu, 4)) != ORN_OK) return st;
         // job splitting: both fields at once (they only pay off together)

@@ -366,14 +366,27 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, float t_min, const
     return trav_internal<F, W>(sc, t_min, world, stack, stride, tr);
 }
 
-// The whole of bvh_hit for one ray.
-template <uint32_t F, uint32_t W>
+// The whole of bvh_hit for one ray. With STEPS > 1 a lane visits up to STEPS
+// internal nodes in a row and then, if it stands at one, a leaf: lanes of a
+// wave that stand at leaves then test them together instead of each leaf
+// visit splitting the wave. Every lane still meets its own nodes in the same
+// order, so the result does not depend on STEPS.
+template <uint32_t F, uint32_t W, uint32_t STEPS = 1>
 __device__ __forceinline__ bool bvh_hit(const DevScene &sc, float t_min, const Ray &world,
                                         uint32_t *stack, uint32_t stride, Hit &h)
 {
     Traversal tr;
     trav_begin(W == 4u ? sc.tlas4_root : sc.tlas2_root, world, tr);
-    while (trav_step<F, W>(sc, t_min, world, stack, stride, tr)) {
+    if constexpr (STEPS <= 1u) {
+        while (trav_step<F, W>(sc, t_min, world, stack, stride, tr)) {
+        }
+    } else {
+        bool on = true;
+        while (on) {
+            for (uint32_t k = 0; k < STEPS && on && !(tr.addr & kLeafRef); k++)
+                on = trav_internal<F, W>(sc, t_min, world, stack, stride, tr);
+            if (on && (tr.addr & kLeafRef)) on = trav_leaf<F, W>(sc, t_min, world, stack, stride, tr);
+        }
     }
     h = tr.h;
     return tr.hit;
@@ -398,7 +411,7 @@ __device__ __forceinline__ bool bvh_hit_packet(const DevScene &sc, float t_min, 
     Traversal tr;
     trav_begin(sc.tlas2_root, world, tr);
     const uint32_t lane = __lane_id();
-    uint64_t active = __ballot(valid);
+    uint64_t active = lanes_where(valid);
     uint32_t addr = sc.tlas2_root;
     bool in_tlas = true;
     uint32_t top = 0;
@@ -420,7 +433,7 @@ __device__ __forceinline__ bool bvh_hit_packet(const DevScene &sc, float t_min, 
                                           tr.ox, t_min, tr.t_max);
             const bool r = me && aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z), tr.inv,
                                           tr.ox, t_min, tr.t_max);
-            const uint64_t ml = __ballot(l), mr = __ballot(r);
+            const uint64_t ml = lanes_where(l), mr = lanes_where(r);
             const uint32_t left = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(as_u32(n0.w))));
             const uint32_t right = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(as_u32(n1.w))));
             if (mr) {
