@@ -174,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     f3 thr = mk(1.0f, 1.0f, 1.0f);
     uint32_t bounce = 0;
     Traversal tr;
+    tr.addr = kLeafRef;  // a lane that is not traversing holds a leaf-flagged reference
     uint32_t mode = kWaiting;
     bool need_pixel = true;
     bool blocked = false;  // holds a job whose predecessor has not finished
@@ -331,20 +332,24 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         // up to `node_steps` internal nodes per lane every iteration; leaves
         // (sphere, triangle, mesh entry: the long code) in rounds of at
         // least `leaf_regroup` lanes
+        // (an unflagged tr.addr implies mode == kTraverse: lanes that stop
+        // traversing take kLeafRef)
         for (uint32_t k = 0; k < node_steps; k++) {
-            if (mode == kTraverse && !(tr.addr & kLeafRef) &&
-                !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr))
+            if (!(tr.addr & kLeafRef) && !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr)) {
                 mode = kWaiting;
+                tr.addr = kLeafRef;
+            }
         }
         // masks of single comparisons, combined on the scalar unit
         const uint64_t tracing = lanes_where(mode == kTraverse);
-        const uint64_t leaf_next = lanes_where((tr.addr & kLeafRef) != 0u);
-        const uint64_t at_leaf = tracing & leaf_next;
-        const uint64_t at_inner = tracing & ~leaf_next;
+        const uint64_t at_inner = lanes_where(!(tr.addr & kLeafRef));
+        const uint64_t at_leaf = tracing & ~at_inner;
         if (at_leaf != 0u && (static_cast<uint32_t>(__popcll(at_leaf)) >= leaf_regroup || at_inner == 0u)) {
             if (mode == kTraverse && (tr.addr & kLeafRef) &&
-                !trav_leaf<F, W>(sc, t_min(), ray, stack, kBlock, tr))
+                !trav_leaf<F, W>(sc, t_min(), ray, stack, kBlock, tr)) {
                 mode = kWaiting;
+                tr.addr = kLeafRef;
+            }
         }
         const uint64_t waiting = lanes_where(mode == kWaiting);
         const uint64_t traversing = lanes_where(mode == kTraverse);

@@ -293,13 +293,10 @@ __device__ __forceinline__ bool trav_internal(const DevScene &sc, float t_min, c
                             tr.ox, t_min, tr.t_max);
     const bool r = aabb_hit(make_float3(n2.x, n2.y, n2.z), make_float3(n3.x, n3.y, n3.z), tr.inv,
                             tr.ox, t_min, tr.t_max);
-    if (r) {
-        if (l && tr.top < cap) stack[(tr.top++) * stride] = as_u32(n0.w);
-        tr.addr = as_u32(n1.w);
-        return true;
-    }
-    if (l) {
-        tr.addr = as_u32(n0.w);
+    // both hit: the left child waits on the stack and the right one is next
+    if (l && r && tr.top < cap) stack[(tr.top++) * stride] = as_u32(n0.w);
+    if (l || r) {
+        tr.addr = r ? as_u32(n1.w) : as_u32(n0.w);
         return true;
     }
     return trav_pop<F>(sc, world, stack, stride, tr);
