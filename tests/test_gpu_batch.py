@@ -78,6 +78,8 @@ def test_batch_matches_oracle_image(device, oracle):
 @pytest.mark.parametrize("tuning", [
     dict(regroup_lanes=8, leaf_regroup_lanes=1, samples_per_job=1, split_percent=100),
     dict(regroup_lanes=64, leaf_regroup_lanes=32, workgroups_per_cu=1, samples_per_job=0),
+    dict(regroup_lanes=16, leaf_regroup_lanes=4, node_steps=16),
+    dict(regroup_lanes=48, node_steps=1, samples_per_job=2, split_percent=50),
 ])
 def test_batch_tunings_give_the_same_images(device, tuning):
     specs = mixed_specs()[:4]
@@ -92,6 +94,7 @@ def test_batch_tunings_give_the_same_images(device, tuning):
                         np.arange(n, dtype=np.uint32) + np.uint32(specs[i].seed * n))
     batch.set_tuning(**tuning)
     assert batch.tuning()["regroup_lanes"] == tuning["regroup_lanes"]
+    assert batch.tuning()["node_steps"] == tuning.get("node_steps", 4)
     batch.render(params, iters)
     for i in range(len(specs)):
         assert same_bits(batch.accumulation(i), ref[i]), specs[i].name

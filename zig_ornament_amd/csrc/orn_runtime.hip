@@ -354,7 +354,12 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         const uint64_t waiting = lanes_where(mode == kWaiting);
         const uint64_t traversing = lanes_where(mode == kTraverse);
         if (waiting == 0u && traversing == 0u) break;
-        if (static_cast<uint32_t>(__popcll(waiting)) < regroup && traversing != 0u) continue;
+        // a shading round once `regroup` lanes wait, or once fewer than half
+        // as many lanes still traverse (a wave then steps few lanes while
+        // most of it waits)
+        const uint32_t n_waiting = static_cast<uint32_t>(__popcll(waiting));
+        if (n_waiting < regroup && traversing != 0u && 2u * static_cast<uint32_t>(__popcll(traversing)) >= n_waiting)
+            continue;
         // shading round
         flush_releases();
         if (mode == kWaiting && !need_pixel) {
