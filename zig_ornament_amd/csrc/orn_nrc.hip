@@ -1194,14 +1194,26 @@ hipError_t refresh_params(orn_nrc *n, hipStream_t st)
 // Forward + backward over records [first, first + batch) of buf (count from
 // the device counter or count_max), then (mode) the gradients stored into
 // mlp_grad and grid_grad, or an Adam step with step count n->step.
-hipError_t gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr, uint32_t count_max, uint32_t first,
-                     uint32_t batch, GradMode mode, hipStream_t st)
+// The first half: forward + backward of the batch (reads the network,
+// writes the per-workgroup partials and the per-sample grid inputs).
+hipError_t train_gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr, uint32_t count_max,
+                           uint32_t first, uint32_t batch, hipStream_t st)
 {
     const uint32_t groups = blocks_for(batch, kTileRows);
     if (groups > n->partial_groups) return hipErrorInvalidValue;
     nrc_train_kernel<<<groups, kBlock, 0, st>>>(n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt,
                                                 buf, count_ptr, count_max, first, batch, n->partials, n->sample_u,
                                                 n->sample_dx, n->train_cap, n->loss);
+    return hipGetLastError();
+}
+
+// The second half: MLP and grid gradients from train_gradients' output,
+// stored or applied (the only kernels of a step that write the network).
+hipError_t apply_gradients(orn_nrc *n, const uint32_t *count_ptr, uint32_t count_max, uint32_t batch, GradMode mode,
+                           hipStream_t st)
+{
+    const uint32_t groups = blocks_for(batch, kTileRows);
+    if (groups > n->partial_groups) return hipErrorInvalidValue;
     const AdamStep a = adam_step(n);
     nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(n->partials, groups, a, n->mlp, n->mlp_grad,
                                                                  n->mlp_m, n->mlp_v, n->mlp_wt, mode);
@@ -1220,6 +1232,14 @@ hipError_t gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr
         nrc_grid_finish_kernel<<<blocks_for(n->parts.finish[kLevels], 256), 256, 0, st>>>(n->grid, n->parts,
                                                                                         n->grid_scratch, up);
     return hipGetLastError();
+}
+
+hipError_t gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *count_ptr, uint32_t count_max, uint32_t first,
+                     uint32_t batch, GradMode mode, hipStream_t st)
+{
+    const hipError_t e = train_gradients(n, buf, count_ptr, count_max, first, batch, st);
+    if (e != hipSuccess) return e;
+    return apply_gradients(n, count_ptr, count_max, batch, mode, st);
 }
 
 // SplitMix64 for the initial parameters (reproducible per seed)
@@ -1544,16 +1564,20 @@ orn_status enqueue_train_paths(orn_nrc *n, const orn_constant_params *params, ui
 }
 
 // `steps` Adam steps on the frame's training records, on the scene's stream.
-orn_status enqueue_steps(orn_nrc *n, uint32_t steps)
+// first_trained: the first step's train_gradients (and the loss reset) have
+// been enqueued already (orn_nrc_frame runs them beside the render).
+orn_status enqueue_steps(orn_nrc *n, uint32_t steps, bool first_trained = false)
 {
     hipStream_t st = n->scene->ctx->stream;
-    nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
+    if (!first_trained) nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
     const uint32_t b = n->cfg.train_batch;
     for (uint32_t k = 0; k < steps; k++) {
         // step k takes records k*b, k*b + 1, ... of this frame's samples,
         // modulo their count (an empty frame makes a step with zero gradient)
         n->step++;
-        ORN_TRY(gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, kUpdate, st));
+        if (k > 0u || !first_trained)
+            ORN_TRY(train_gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
+        ORN_TRY(apply_gradients(n, n->counters + 1, n->train_cap, b, kUpdate, st));
     }
     return ORN_OK;
 }
@@ -1617,10 +1641,16 @@ orn_status orn_nrc_frame(orn_nrc *n, orn_target *t, const orn_constant_params *p
     ORN_TRY(hipEventRecord(n->fork, st));
     ORN_TRY(hipStreamWaitEvent(n->side, n->fork, 0));
     if ((r = enqueue_train_paths(n, params, frame, n->side)) != ORN_OK) return r;
+    // the first step's forward + backward only reads the network, so it runs
+    // beside the render too; the kernels that write the network wait for both
+    if (steps > 0u) {
+        nrc_zero_kernel<<<1, 32, 0, n->side>>>(n->loss, 4);
+        ORN_TRY(train_gradients(n, n->train_recs, n->counters + 1, n->train_cap, 0u, n->cfg.train_batch, n->side));
+    }
     ORN_TRY(hipEventRecord(n->join, n->side));
     if ((r = enqueue_render(n, t, params, iterations)) != ORN_OK) return r;
-    ORN_TRY(hipStreamWaitEvent(st, n->join, 0));  // the steps read the records; the render read the weights
-    if ((r = enqueue_steps(n, steps)) != ORN_OK) return r;
+    ORN_TRY(hipStreamWaitEvent(st, n->join, 0));  // the updates read the gradients; the render read the weights
+    if ((r = enqueue_steps(n, steps, steps > 0u)) != ORN_OK) return r;
     ORN_TRY(hipEventRecord(n->frame_stop, st));
     n->framed = true;
     if (loss) {
