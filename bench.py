@@ -22,7 +22,11 @@ Also reported:
                the stream the kernel runs on), against 8 TB/s of HBM3E;
                `traffic` is the HBM bytes per launch that rocprofv3's
                FETCH_SIZE and WRITE_SIZE counters measured for this command,
-               read from the committed profile summary.
+               read from the committed profile summary. The fraction can pass
+               1 because those bytes are served from on-chip caches; what
+               bounds the kernel is vector-instruction issue, whose share of
+               SIMD cycles (SQ counters, committed summary) is reported as
+               `valu_issue_frac`.
   cpu_baseline the oracle (a C restatement of the WGSL megakernel, OpenMP
                over pixels, serial samples per pixel) timed on the host on a
                bounded sample of the same workload.
@@ -63,17 +67,24 @@ WORKLOADS = {
     "C3": ("instanced_icospheres_1920x1080_32spp_depth8", "1,024 instanced icospheres 1920x1080 32spp depth 8"),
 }
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "render_kernel_random_spheres_1080p_64spp_pmc.md")
+ISSUE_PROFILE = os.path.join(ROOT, "profiles", "render_kernel_random_spheres_1080p_64spp_sq.md")
+
+
+def profiled_figure(path, name, kind=float):
+    """A figure (`name = N`) from a committed rocprofv3 counter summary, or
+    None."""
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(name + r"\s*=\s*([0-9.]+)", text)
+    return kind(m.group(1)) if m else None
 
 
 def profiled_traffic():
-    """HBM bytes per render launch from the committed rocprofv3 counter
-    summary (`traffic_bytes_per_launch = N`), or None."""
-    try:
-        text = open(TRAFFIC_PROFILE).read()
-    except OSError:
-        return None
-    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9]+)", text)
-    return int(m.group(1)) if m else None
+    """HBM bytes per render launch (FETCH_SIZE + WRITE_SIZE, scaled by a
+    plain copy kernel), or None."""
+    return profiled_figure(TRAFFIC_PROFILE, "traffic_bytes_per_launch", int)
 
 
 def algorithmic_bytes(stats):
@@ -335,6 +346,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                            "traffic": profiled_traffic() if args.workload == "C2" else None,
+                           "valu_issue_frac": (profiled_figure(ISSUE_PROFILE, "valu_issue_fraction")
+                                               if args.workload == "C2" else None),
                            "algorithmic_bytes_per_sample": cb["bytes_per_sample"],
                            "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
                            "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"],
