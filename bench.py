@@ -267,9 +267,10 @@ def main():
                          "same line for the Cornell box and the instanced icospheres)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the path tracer shards by pixel tiles with no "
-              "communication during rendering, and this build runs on a single MI355X. "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the path tracer would shard an image by pixel tiles, "
+              "with no communication while rendering and one gather of the framebuffer at the "
+              "end, which is not implemented; it runs on a single MI355X. Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     spec = W.CONFIGS[args.workload]()
