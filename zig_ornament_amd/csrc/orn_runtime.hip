@@ -370,24 +370,32 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                 }
             } else {
                 bounce++;
+                // each lane's outcome first, then one next_traversal for all
+                // of them (inlined once, so lanes that missed and lanes that
+                // scattered share its code)
+                bool done;
+                f3 color;
                 if (!tr.hit) {
                     const f3 ud = normalize(ray.d);
                     const float tt = 0.5f * (ud.y + 1.0f);
                     const float a = (1.0f - tt) * 1.0f;
-                    next_traversal(true, thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f));
+                    done = true;
+                    color = thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f);
                 } else {
                     Ray next;
                     f3 w;
                     const uint32_t tex_count = params().textures_count;
                     const uint32_t tex_base = BATCH ? items[item].tex_base : 0u;
-                    if (shade<F>(sc, tex_count, tex_base, ray, tr.h, rng, next, w)) {
+                    done = !shade<F>(sc, tex_count, tex_base, ray, tr.h, rng, next, w);
+                    if (done) {
+                        color = thr * w;
+                    } else {
                         ray = next;
                         thr = thr * w;
-                        next_traversal(false, thr);
-                    } else {
-                        next_traversal(true, thr * w);
+                        color = thr;
                     }
                 }
+                next_traversal(done, color);
             }
         }
         // lanes without a job take the next ones of the wave's band, one
