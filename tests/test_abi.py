@@ -147,3 +147,22 @@ def test_scene_api_counts_and_camera():
     # createSphereMesh: 60 x 30 UV sphere (scene.zig:140-233)
     b = zo.Bvh(s)
     assert len(b.normals) == 4 + (1 + 29 * 60 + 1)
+
+
+def test_shipped_tuning_cache_entries_are_valid():
+    """Every entry of the shipped cache names a scene class the library
+    builds keys for and holds a tuning orn_target_set_tuning accepts (ranges
+    of orn_tuning in include/ornament_hip.h); an invalid entry would be
+    skipped and the class measured again."""
+    import json
+    cache = json.load(open(zo.TUNING_CACHE))
+    assert len(cache) >= 4
+    key = re.compile(r"^gfx[0-9a-f]+[^|]*\|features[0-7]\|tlas2\^\d+\|blas2\^\d+\|px2\^\d+\|spp2\^\d+\|depth\d+"
+                     r"\|images2\^\d+\|bvh4[01]$")
+    for k, e in cache.items():
+        assert key.match(k), k
+        assert 1 <= e["regroup_lanes"] <= 64 and 1 <= e["leaf_regroup_lanes"] <= 64, k
+        assert e["workgroups_per_cu"] <= 64 and e["bvh_width"] in (2, 4), k
+        assert e["samples_per_job"] <= 0xFFFF and e["split_percent"] <= 100, k
+        assert 0 <= e.get("node_steps", 0) <= 16, k
+        assert e["ms"] > 0, k
