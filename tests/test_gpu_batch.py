@@ -138,6 +138,13 @@ def test_target_autotune_keeps_state_and_uses_cache(device, tmp_path):
     t2 = zo.Target(ds, spec.width, spec.height, 0)
     chosen2, ms2 = t2.autotune(p, 2, cache_path=cache)
     assert chosen2 == chosen and abs(ms2 - entries[key]["ms"]) < 1e-3
+    assert entries[key]["node_steps"] == chosen["node_steps"]
+    # an entry written before node_steps existed reads as the default (4)
+    del entries[key]["node_steps"]
+    old = str(tmp_path / "old.json")
+    json.dump(entries, open(old, "w"))
+    chosen_old, _ = zo.Target(ds, spec.width, spec.height, 0).autotune(p, 2, cache_path=old)
+    assert chosen_old == dict(chosen, node_steps=4)
     # and renders afterwards still match the untuned render
     t2.render(p, 2)
     t3 = zo.Target(ds, spec.width, spec.height, 0)
