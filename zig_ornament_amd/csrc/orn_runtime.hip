@@ -331,9 +331,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     for (;;) {
         // up to `node_steps` internal nodes per lane every iteration; leaves
         // (sphere, triangle, mesh entry: the long code) in rounds of at
-        // least `leaf_regroup` lanes
-        // (an unflagged tr.addr implies mode == kTraverse: lanes that stop
-        // traversing take kLeafRef)
+        // least `leaf_regroup` lanes. An unflagged tr.addr implies
+        // mode == kTraverse: lanes that stop traversing take kLeafRef.
         for (uint32_t k = 0; k < node_steps; k++) {
             if (!(tr.addr & kLeafRef) && !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr)) {
                 mode = kWaiting;
@@ -355,8 +354,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         const uint64_t traversing = lanes_where(mode == kTraverse);
         if (waiting == 0u && traversing == 0u) break;
         // a shading round once `regroup` lanes wait, or once fewer than half
-        // as many lanes still traverse (a wave then steps few lanes while
-        // most of it waits)
+        // as many lanes still traverse as wait (the wave would otherwise step
+        // a few lanes while most of it idles)
         const uint32_t n_waiting = static_cast<uint32_t>(__popcll(waiting));
         if (n_waiting < regroup && traversing != 0u && 2u * static_cast<uint32_t>(__popcll(traversing)) >= n_waiting)
             continue;
