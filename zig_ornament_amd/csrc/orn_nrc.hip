@@ -42,6 +42,10 @@ namespace {
 constexpr uint32_t kBlock = 256;
 // internal BVH nodes a lane visits in a row before a leaf visit (bvh_hit)
 constexpr uint32_t kNodeSteps = 4;
+// nrc_grid_kernel: table entries per workgroup (64 KB of LDS, two feature
+// sums per entry) and lanes per workgroup
+constexpr uint32_t kPartEntries = 8192;
+constexpr uint32_t kGridBlock = 1024;
 constexpr uint32_t kIn = 64;     // network input width
 constexpr uint32_t kW = 64;      // hidden width
 constexpr uint32_t kOut = 16;    // output width (RGB + padding)
@@ -363,9 +367,12 @@ __global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __ha
 // record count (a frame with fewer records than a step needs reuses them;
 // none: an empty step). Writes the tile's weight gradient to
 // partials[blockIdx.x], the tile's loss to loss_sum, and for batch sample b
-// its unit position (sample_u[b]) and the loss gradient of its grid features
-// (sample_dx[l * dx_stride + b], both features of level l), from which
-// nrc_grid_kernel forms the grid gradient without global atomics.
+// its unit position (sample_u[b]), the loss gradient of its grid features
+// (sample_dx[l * dx_stride + b], both features of level l) and the ranges
+// of kPartEntries table entries its 8 corners at level l fall in
+// (sample_mask[l * dx_stride + b]: bit (range % 64) of each; 0 when the
+// gradient is 0), from which nrc_grid_kernel forms the grid gradient without
+// global atomics.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __half2 *__restrict__ table,
                                                            const uint16_t *__restrict__ wt,
@@ -373,7 +380,8 @@ __global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __ha
                                                            const uint32_t *__restrict__ count_ptr, uint32_t count_max,
                                                            uint32_t first, uint32_t batch,
                                                            float *__restrict__ partials, float4 *__restrict__ sample_u,
-                                                           float2 *__restrict__ sample_dx, uint32_t dx_stride,
+                                                           float2 *__restrict__ sample_dx,
+                                                           uint64_t *__restrict__ sample_mask, uint32_t dx_stride,
                                                            float *__restrict__ loss_sum)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtElems];
@@ -459,14 +467,21 @@ __global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __ha
         const uint32_t s = threadIdx.x % kTileRows, part4 = threadIdx.x / kTileRows;
         if (s < valid) {
             const uint32_t b = tile_rel + s;
-            if (part4 == 0u) {
-                float u[3];
-                unit_position(g, recs[tile_record(tile_base, s, total)].pos, u);
-                sample_u[b] = make_float4(u[0], u[1], u[2], 0.0f);
+            float u[3];
+            unit_position(g, recs[tile_record(tile_base, s, total)].pos, u);
+            if (part4 == 0u) sample_u[b] = make_float4(u[0], u[1], u[2], 0.0f);
+            for (uint32_t l = part4 * 4u; l < part4 * 4u + 4u; l++) {
+                const float2 d = make_float2(s_dx0[s * kGridFeatures + 2u * l], s_dx0[s * kGridFeatures + 2u * l + 1u]);
+                sample_dx[size_t(l) * dx_stride + b] = d;
+                uint64_t m = 0;
+                if (d.x != 0.0f || d.y != 0.0f) {
+                    uint32_t idx[8];
+                    float w[8];
+                    level_corners(g, l, u, idx, w);
+                    for (int c = 0; c < 8; c++) m |= 1ull << ((idx[c] / kPartEntries) & 63u);
+                }
+                sample_mask[size_t(l) * dx_stride + b] = m;
             }
-            for (uint32_t l = part4 * 4u; l < part4 * 4u + 4u; l++)
-                sample_dx[size_t(l) * dx_stride + b] =
-                    make_float2(s_dx0[s * kGridFeatures + 2u * l], s_dx0[s * kGridFeatures + 2u * l + 1u]);
         }
     }
 }
@@ -553,7 +568,11 @@ __global__ void nrc_grid_half_kernel(const float *__restrict__ w, __half *__rest
 // kPartEntries consecutive table entries of one level and one chunk of the
 // batch, and sums into LDS the contributions w_c * dL/df of the chunk's
 // samples whose cell has a corner in its range (sample b: unit position
-// sample_u[b], feature gradients sample_dx[l * dx_stride + b]). Levels whose
+// sample_u[b], feature gradients sample_dx[l * dx_stride + b]). Each lane
+// first reads the range masks (sample_mask[l * dx_stride + b]) of 64 of the
+// chunk's samples and keeps, in a register, which of them can touch the
+// workgroup's range; it hashes and sums only those (about an eighth of them
+// at a hashed level, nearly all at a dense one). Levels whose
 // table is small next to the others (the coarse, densely indexed ones, where
 // nearly every corner lands in one range) split the batch into chunks so
 // that every workgroup sums about as many contributions; their chunk sums
@@ -563,8 +582,6 @@ __global__ void nrc_grid_half_kernel(const float *__restrict__ w, __half *__rest
 // and moments, as in Mueller et al. 2022, sec. 4: with a table much larger
 // than the batch most entries are untouched), and rewrites their fp16 copies.
 // The LDS sums take a parameter's contributions in no fixed order.
-constexpr uint32_t kPartEntries = 8192;  // 64 KB of LDS (two feature sums per entry)
-constexpr uint32_t kGridBlock = 1024;
 
 struct GridParts {
     uint32_t first[kLevels + 1];  // first workgroup of each level; first[kLevels] = all workgroups
@@ -602,7 +619,9 @@ __device__ __forceinline__ void finish_entry(const GridUpdate &u, size_t j, floa
 
 __global__ __launch_bounds__(kGridBlock) void nrc_grid_kernel(NrcGrid g, GridParts parts,
                                                               const float4 *__restrict__ sample_u,
-                                                              const float2 *__restrict__ sample_dx, uint32_t dx_stride,
+                                                              const float2 *__restrict__ sample_dx,
+                                                              const uint64_t *__restrict__ sample_mask,
+                                                              uint32_t dx_stride,
                                                               const uint32_t *__restrict__ count_ptr, uint32_t count_max,
                                                               uint32_t batch, float2 *__restrict__ scratch, GridUpdate up)
 {
@@ -619,26 +638,37 @@ __global__ __launch_bounds__(kGridBlock) void nrc_grid_kernel(NrcGrid g, GridPar
     const uint32_t per = (n_batch + chunks - 1u) / chunks;
     const uint32_t b_end = min(n_batch, (chunk + 1u) * per);
     const float2 *dx = sample_dx + size_t(l) * dx_stride;
-    constexpr uint32_t kAhead = 4;  // samples per thread whose loads are in flight together
-    for (uint32_t b0 = chunk * per + threadIdx.x; b0 < b_end; b0 += kAhead * kGridBlock) {
-        float2 d[kAhead];
-        float4 uu[kAhead];
-        for (uint32_t k = 0; k < kAhead; k++) {
-            const uint32_t b = b0 + k * kGridBlock;
-            d[k] = b < b_end ? dx[b] : make_float2(0.0f, 0.0f);
-            uu[k] = b < b_end ? sample_u[b] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint64_t *msk = sample_mask + size_t(l) * dx_stride;
+    const uint64_t bit = 1ull << ((e0 / kPartEntries) & 63u);
+    constexpr uint32_t kScan = 64;   // samples per lane whose masks are read before any is summed
+    constexpr uint32_t kAhead = 8;   // mask loads in flight together
+    for (uint32_t b0 = chunk * per + threadIdx.x; b0 < b_end; b0 += kScan * kGridBlock) {
+        uint64_t hits = 0;  // bit i: sample b0 + i * kGridBlock may touch the range
+#pragma unroll 1
+        for (uint32_t i0 = 0; i0 < kScan; i0 += kAhead) {
+            uint64_t m[kAhead];
+            for (uint32_t k = 0; k < kAhead; k++) {
+                const uint32_t b = b0 + (i0 + k) * kGridBlock;
+                m[k] = b < b_end ? msk[b] : 0ull;
+            }
+            for (uint32_t k = 0; k < kAhead; k++)
+                if (m[k] & bit) hits |= 1ull << (i0 + k);
         }
-        for (uint32_t k = 0; k < kAhead; k++) {
-            if (d[k].x == 0.0f && d[k].y == 0.0f) continue;
-            const float u[3] = {uu[k].x, uu[k].y, uu[k].z};
+        while (hits != 0u) {
+            const uint32_t i = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(hits))) - 1u;
+            hits &= hits - 1u;
+            const uint32_t b = b0 + i * kGridBlock;
+            const float2 d = dx[b];
+            const float4 uu = sample_u[b];
+            const float u[3] = {uu.x, uu.y, uu.z};
             uint32_t idx[8];
             float wc[8];
             level_corners(g, l, u, idx, wc);
             for (int c = 0; c < 8; c++) {
                 const uint32_t e = idx[c] - e0;
                 if (e < span) {
-                    if (d[k].x != 0.0f) atomicAdd(&s_acc[e].x, wc[c] * d[k].x);
-                    if (d[k].y != 0.0f) atomicAdd(&s_acc[e].y, wc[c] * d[k].y);
+                    if (d.x != 0.0f) atomicAdd(&s_acc[e].x, wc[c] * d.x);
+                    if (d.y != 0.0f) atomicAdd(&s_acc[e].y, wc[c] * d.y);
                 }
             }
         }
@@ -1086,6 +1116,7 @@ struct orn_nrc {
     uint32_t partial_groups;
     float4 *sample_u;   // per batch sample: unit position
     float2 *sample_dx;  // per level and batch sample: loss gradient of the two grid features
+    uint64_t *sample_mask;  // per level and batch sample: nrc_grid_kernel ranges its corners reach
     GridParts parts;    // nrc_grid_kernel's workgroups per level
     float2 *grid_scratch;  // chunk sums of the chunked levels
     NrcRecord *train_recs;
@@ -1151,7 +1182,7 @@ void free_nrc(orn_nrc *n)
     (void)hipStreamSynchronize(n->scene->ctx->stream);
     void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
                     n->grid_grad, n->grid_half, n->partials, n->train_recs, n->queries, n->partial, n->counters,
-                    n->loss, n->sample_u, n->sample_dx, n->grid_scratch, n->train_stage, n->train_keep,
+                    n->loss, n->sample_u, n->sample_dx, n->sample_mask, n->grid_scratch, n->train_stage, n->train_keep,
                     n->train_block_keep, n->train_offsets};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1203,7 +1234,7 @@ hipError_t train_gradients(orn_nrc *n, const NrcRecord *buf, const uint32_t *cou
     if (groups > n->partial_groups) return hipErrorInvalidValue;
     nrc_train_kernel<<<groups, kBlock, 0, st>>>(n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt,
                                                 buf, count_ptr, count_max, first, batch, n->partials, n->sample_u,
-                                                n->sample_dx, n->train_cap, n->loss);
+                                                n->sample_dx, n->sample_mask, n->train_cap, n->loss);
     return hipGetLastError();
 }
 
@@ -1226,8 +1257,8 @@ hipError_t apply_gradients(orn_nrc *n, const uint32_t *count_ptr, uint32_t count
     up.half_w = n->grid_half;
     up.mode = mode;
     nrc_grid_kernel<<<n->parts.first[kLevels], kGridBlock, 0, st>>>(n->grid, n->parts, n->sample_u, n->sample_dx,
-                                                                    n->train_cap, count_ptr, count_max, batch,
-                                                                    n->grid_scratch, up);
+                                                                    n->sample_mask, n->train_cap, count_ptr,
+                                                                    count_max, batch, n->grid_scratch, up);
     if (n->parts.finish[kLevels])
         nrc_grid_finish_kernel<<<blocks_for(n->parts.finish[kLevels], 256), 256, 0, st>>>(n->grid, n->parts,
                                                                                         n->grid_scratch, up);
@@ -1349,6 +1380,7 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         (e = hipMalloc(&n->train_offsets, size_t(blocks_for(std::max(cfg->train_paths, 1u), kBlock)) * 4u)) ||
         (e = hipMalloc(&n->sample_u, size_t(n->train_cap) * sizeof(float4))) ||
         (e = hipMalloc(&n->sample_dx, size_t(n->train_cap) * kLevels * sizeof(float2))) ||
+        (e = hipMalloc(&n->sample_mask, size_t(n->train_cap) * kLevels * sizeof(uint64_t))) ||
         (scratch_entries && (e = hipMalloc(&n->grid_scratch, size_t(scratch_entries) * sizeof(float2)))) ||
         (e = hipMalloc(&n->counters, 16)) || (e = hipMalloc(&n->loss, 16)))
         return fail(e);
