@@ -2172,6 +2172,28 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
         if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
         if (!(cur_ms < before)) break;
     }
+    // Then the settings between the coarse candidates around the best one:
+    // regroup_lanes +-4, leaf_regroup_lanes +-2 and +-4, node_steps +-1.
+    auto nudge = [&](uint32_t orn_tuning::*field, int delta, int lo, int hi) -> orn_status {
+        const int v = static_cast<int>(cur.*field) + delta;
+        if (v < lo || v > hi) return ORN_OK;
+        orn_tuning t = cur;
+        t.*field = static_cast<uint32_t>(v);
+        float ms = 0;
+        const orn_status s = measure(t, &ms);
+        if (s != ORN_OK) return s;
+        if (ms < cur_ms) {
+            cur = t;
+            cur_ms = ms;
+        }
+        return ORN_OK;
+    };
+    for (int d : {-4, 4})
+        if ((st = nudge(&orn_tuning::regroup_lanes, d, 1, 64)) != ORN_OK) return st;
+    for (int d : {-4, -2, 2, 4})
+        if ((st = nudge(&orn_tuning::leaf_regroup_lanes, d, 1, 64)) != ORN_OK) return st;
+    for (int d : {-1, 1})
+        if ((st = nudge(&orn_tuning::node_steps, d, 1, 16)) != ORN_OK) return st;
     *best = cur;
     *best_ms = cur_ms;
     return ORN_OK;
