@@ -322,9 +322,10 @@ orn_status orn_batch_set_state(orn_batch *batch, size_t item, const float *accum
 
 /* Times renders of `iterations` samples per pixel (a few suffice) under
  * candidate tunings (coordinate descent over regroup_lanes,
- * leaf_regroup_lanes, workgroups_per_cu, split_percent, samples_per_job and,
- * with ORN_TUNE_ALLOW_BVH4, bvh_width), sets the fastest on the target and
- * returns it with its time.
+ * leaf_regroup_lanes, node_steps, workgroups_per_cu, split_percent,
+ * samples_per_job and, with ORN_TUNE_ALLOW_BVH4, bvh_width; then the
+ * neighbours of the best regroup_lanes, leaf_regroup_lanes and node_steps),
+ * sets the fastest on the target and returns it with its time.
  * The target's accumulation, RNG state and framebuffer are left as they
  * were. With a cache_path (a JSON file, created if absent), a tuning stored
  * for the same device architecture and scene class (scene features, BVH
