@@ -184,6 +184,7 @@ enum SceneFeatures : uint32_t {
 struct Traversal {
     Ray ray;        // current space (world, or object space inside a BLAS)
     f3 inv, ox;     // safe_invdir(ray.d) and -ray.o * inv
+    f3 winv, wox;   // the same of the world-space ray, restored on leaving a BLAS
     float t_max;
     uint32_t addr;  // next node to visit
     uint32_t top;   // pending entries in the LDS column
@@ -199,6 +200,8 @@ __device__ __forceinline__ void trav_begin(uint32_t root, const Ray &world, Trav
     tr.ray = world;
     tr.inv = safe_invdir(world.d);
     tr.ox = (-world.o) * tr.inv;
+    tr.winv = tr.inv;
+    tr.wox = tr.ox;
     tr.t_max = kTMaxInit;
     tr.addr = root;
     tr.top = 0u;
@@ -224,12 +227,12 @@ __device__ __forceinline__ bool trav_pop(const DevScene &sc, const Ray &world, u
     if (tr.top == 0u) return false;
     tr.addr = stack[(--tr.top) * stride];
     if ((F & kHasMeshes) && tr.addr == kFinishedTraverseBlas) {
-        // back in the TLAS: the world-space ray and its reciprocal are
-        // recomputed (the same operations give the same values)
+        // back in the TLAS: the world-space ray and its reciprocal, kept
+        // from trav_begin
         tr.in_tlas = true;
         tr.ray = world;
-        tr.inv = safe_invdir(world.d);
-        tr.ox = (-world.o) * tr.inv;
+        tr.inv = tr.winv;
+        tr.ox = tr.wox;
         if (tr.top == 0u) return false;
         tr.addr = stack[(--tr.top) * stride];
     }
@@ -502,8 +505,8 @@ __device__ __forceinline__ bool bvh_hit_packet(const DevScene &sc, float t_min, 
             if ((F & kHasMeshes) && a == kFinishedTraverseBlas) {
                 in_tlas = true;
                 tr.ray = world;
-                tr.inv = safe_invdir(world.d);
-                tr.ox = (-world.o) * tr.inv;
+                tr.inv = tr.winv;
+                tr.ox = tr.wox;
                 continue;
             }
             addr = a;
