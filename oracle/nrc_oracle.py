@@ -1,7 +1,7 @@
 """CPU oracle of the neural radiance cache (TEST INFRASTRUCTURE ONLY: tests/
 import it as the checker; the product never does).
 
-The cache is an addition of this build (the reference has no learned part),
+The cache is an addition to the reference (which has no learned part),
 so the oracle restates the method as include/ornament_hip.h defines it, with
 the same rounding points as the gfx950 kernels:
 

@@ -3,7 +3,7 @@
 // The reference keeps this layer in Zig: scene factories (src/scene.zig),
 // camera (src/camera.zig), render state (src/state.zig), the TLAS/BLAS build
 // (src/bvh.zig) and the backend driver loop (wgpu_backend/path_tracer.zig).
-// No Zig toolchain is available to this build, so it is restated here with
+// Here it is restated in C++ (no Zig toolchain is assumed) with
 // the same names, argument meaning and error behaviour, and drives the gfx950
 // kernels through include/ornament_hip.h.
 //
@@ -314,7 +314,7 @@ namespace {
 
 // std.rand.DefaultPrng of Zig 0.11: Xoshiro256 (xoshiro256++) seeded by
 // SplitMix64. The reference shares one generator seeded with 1244 across all
-// builds in a process (bvh.zig:12-13); this build reseeds it per Bvh.init so
+// builds in a process (bvh.zig:12-13); this layer reseeds it per Bvh.init so
 // a build does not depend on what was built before it in the same process.
 struct Xoshiro256 {
     uint64_t s[4];
