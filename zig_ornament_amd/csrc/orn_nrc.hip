@@ -41,7 +41,7 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 // internal BVH nodes a lane visits in a row before a leaf visit (bvh_hit)
-constexpr uint32_t kNodeSteps = 2;
+constexpr uint32_t kNodeSteps = 4;
 // nrc_grid_kernel: table entries per workgroup (64 KB of LDS, two feature
 // sums per entry) and lanes per workgroup
 constexpr uint32_t kPartEntries = 8192;
