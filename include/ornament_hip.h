@@ -110,7 +110,12 @@ typedef struct orn_constant_params {
 
 /* One texture as ornament.Texture holds it (texture.zig:4-38). Supported
  * formats (wgpu_backend/buffers.zig:316-336): 8-bit unorm with 1, 2 or 4
- * components, and 16-bit float (is_hdr) with 1, 2 or 4 components. */
+ * components, and 16-bit float (is_hdr) with 1, 2 or 4 components. Width and
+ * height are at most ORN_MAX_TEXTURE_DIMENSION, WebGPU's default
+ * maxTextureDimension2D (the reference requests its device without raised
+ * limits, wgpu_backend/device_state.zig:78-82); larger textures are rejected
+ * with ORN_ERROR_UNSUPPORTED_TEXTURE. */
+#define ORN_MAX_TEXTURE_DIMENSION 8192u
 typedef struct orn_texture_desc {
     uint32_t width;
     uint32_t height;

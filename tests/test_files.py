@@ -107,6 +107,35 @@ def test_corrupt_scene_files_are_rejected(tmp_path):
     assert e.value.name == "IoError"
 
 
+def _texture_only_scene(w, h, comps, bpc, data):
+    """A scene file with no camera settings and one texture record."""
+    head = b"ORNSCENE" + struct.pack("<II", 1, 0) + struct.pack("<13f", *([0.0] * 13))
+    head += struct.pack("<5I", 1, 0, 0, 0, 0)
+    rec = struct.pack("<5IfQ", w, h, comps, bpc, 1 if bpc == 2 else 0, 1.0, len(data)) + data
+    return head + rec + b"\0" * (-len(rec) % 4)
+
+
+def test_texture_sizes_that_overflow_are_rejected(tmp_path):
+    """4294705160 x 536903681 RGBA16F is 64 bytes modulo 2^64: without a bound
+    on the dimensions it would pass the size check and the kernel would index
+    texels 2^61 bytes past a 64-byte pool."""
+    assert (4294705160 * 536903681 * 4 * 2) % 2**64 == 64
+    p = str(tmp_path / "overflow.orns")
+    open(p, "wb").write(_texture_only_scene(4294705160, 536903681, 4, 2, bytes(64)))
+    with pytest.raises(zo.OrnamentError) as e:
+        zo.Scene.load(p)
+    assert e.value.name == "InvalidFile"
+    # the bound itself: WebGPU's default maxTextureDimension2D, 8192
+    s = zo.Scene()
+    with pytest.raises(zo.OrnamentError) as e:
+        s.create_texture(np.zeros(8193 * 4, np.uint8), 8193, 1, 4)
+    assert e.value.name == "UnsupportedTexture"
+    s.create_texture(np.zeros(8192 * 4, np.uint8), 8192, 1, 4)
+    # a well-formed texture-only file loads
+    open(p, "wb").write(_texture_only_scene(2, 2, 4, 1, bytes(16)))
+    zo.Scene.load(p)
+
+
 def test_pfm_layout_and_round_trip(tmp_path):
     rng = np.random.default_rng(1)
     w, h = 5, 3

@@ -940,6 +940,10 @@ orn_status ornament_scene_create_texture(ornament_scene *s, const uint8_t *data,
                                          int is_hdr, float gamma, ornament_texture **out)
 {
     if (!s) return ORN_ERROR_INVALID_HANDLE;
+    // bounded sizes keep every product below 2^32 (8192 * 8192 * 4 * 8)
+    if (width > ORN_MAX_TEXTURE_DIMENSION || height > ORN_MAX_TEXTURE_DIMENSION || num_components > 4u ||
+        bytes_per_component > 8u)
+        return ORN_ERROR_UNSUPPORTED_TEXTURE;
     const size_t bytes_per_row = size_t(width) * num_components * bytes_per_component;
     if (!data || !out || data_len < bytes_per_row * height) return ORN_ERROR_INVALID_VALUE;
     auto t = std::make_unique<ornament_texture>();
@@ -1454,7 +1458,8 @@ orn_status ornament_scene_load(const char *path, ornament_scene **out, ornament_
         if (!r.get_array(n, data)) return bad();
         while (r.at % 4 && r.ok) (void)r.get<uint8_t>();
         ornament_texture *t = nullptr;
-        if (uint64_t(wdt) * hgt * comps * bpc != n ||
+        if (wdt > ORN_MAX_TEXTURE_DIMENSION || hgt > ORN_MAX_TEXTURE_DIMENSION || comps > 4u || bpc > 8u ||
+            uint64_t(wdt) * hgt * comps * bpc != n ||
             ornament_scene_create_texture(s, data.data(), data.size(), wdt, hgt, comps, bpc, int(hdr), gamma, &t))
             return bad();
         tex.push_back(t);

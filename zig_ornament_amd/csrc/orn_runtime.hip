@@ -771,7 +771,9 @@ orn_status validate(const orn_scene_desc *d, uint32_t *stack_slots)
         const orn_texture_desc &t = d->textures[i];
         const bool comps_ok = t.num_components == 1 || t.num_components == 2 || t.num_components == 4;
         const bool fmt_ok = t.is_hdr ? t.bytes_per_component == 2 : t.bytes_per_component == 1;
-        if (!comps_ok || !fmt_ok || t.width == 0 || t.height == 0) return ORN_ERROR_UNSUPPORTED_TEXTURE;
+        if (!comps_ok || !fmt_ok || t.width == 0 || t.height == 0 || t.width > ORN_MAX_TEXTURE_DIMENSION ||
+            t.height > ORN_MAX_TEXTURE_DIMENSION)
+            return ORN_ERROR_UNSUPPORTED_TEXTURE;
         if (!t.data) return ORN_ERROR_INVALID_VALUE;
     }
     // trees deeper than the reference's 64-entry stack are rejected before
