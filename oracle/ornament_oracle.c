@@ -29,6 +29,13 @@
  *   slab/triangle/sphere tests   pinned: closed forms + FP64 brute force
  *   bvh_hit                      pinned: FP64 brute force over primitives
  *   materials / ray_color        pinned: furnace tests, draw counts, closed forms
+ *   dielectric / metal scatter   pinned: Snell and mirror laws, Schlick
+ *                                decision vs the draw, TIR without a draw,
+ *                                absorption below the surface
+ *   texture fetch, half floats   pinned: exact nearest/clamp texel selection,
+ *                                byte/255, all 65,536 binary16 encodings
+ *   sphere uv, mesh normal       pinned: axis-point closed forms; inverse-
+ *                                transpose normal of a non-uniform instance
  *   BVH build node order         parity unpinned (needs Zig 0.11 std to run);
  *                                invariants pinned
  *   zmath host arithmetic        parity unpinned (zmath is not vendored);
@@ -1023,6 +1030,42 @@ static v3 material_emit(const orc_scene *sc, const orc_constant_params *p, const
     return mk(0.0f, 0.0f, 0.0f);
 }
 
+/* pathtracer.wgsl:111-144: the hit record of a closest hit */
+static void make_hit_record(const orc_scene *sc, v3 ro, v3 rd, const orc_hit *hh, hit_record *hit,
+                            orc_stats *st)
+{
+    uint32_t transform_id = hh->inverted_transform_id + 1u;
+    memset(hit, 0, sizeof(*hit));
+    hit->t = hh->t;
+    hit->p = vadd(ro, vscale(hh->t, rd));
+    hit->material_index = hh->material_index;
+    if (hh->node_type == 1u) {
+        if (st) { st->sphere_hits++; }
+        v3 center = xform(&sc->transforms[16u * transform_id], mk(0.0f, 0.0f, 0.0f), 1.0f);
+        v3 on = vnormalize(vsub(hit->p, center));
+        float theta = orc_acosf(-on.y);
+        float phi = orc_atan2f(-on.z, on.x) + PI_F;
+        hit->uv[0] = phi / (2.0f * PI_F);
+        hit->uv[1] = theta / PI_F;
+        set_face_normal(hit, rd, on);
+    } else if (hh->node_type == 2u) {
+        if (st) { st->mesh_hits++; }
+        uint32_t tri = hh->tri_id;
+        v3 n0 = vload(&sc->normals[4u * sc->normal_indices[tri]]);
+        v3 n1 = vload(&sc->normals[4u * sc->normal_indices[tri + 1u]]);
+        v3 n2 = vload(&sc->normals[4u * sc->normal_indices[tri + 2u]]);
+        const float *uv0 = &sc->uvs[2u * sc->uv_indices[tri]];
+        const float *uv1 = &sc->uvs[2u * sc->uv_indices[tri + 1u]];
+        const float *uv2 = &sc->uvs[2u * sc->uv_indices[tri + 2u]];
+        float w = 1.0f - hh->uv[0] - hh->uv[1];
+        v3 normal = vadd(vadd(vscale(w, n0), vscale(hh->uv[0], n1)), vscale(hh->uv[1], n2));
+        hit->uv[0] = w * uv0[0] + hh->uv[0] * uv1[0] + hh->uv[1] * uv2[0];
+        hit->uv[1] = w * uv0[1] + hh->uv[0] * uv1[1] + hh->uv[1] * uv2[1];
+        v3 on = vnormalize(xform_normal(&sc->transforms[16u * hh->inverted_transform_id], normal));
+        set_face_normal(hit, rd, on);
+    }
+}
+
 /* pathtracer.wgsl:95-161: depth exhaustion returns the throughput */
 static v3 ray_color(const orc_scene *sc, const orc_constant_params *p, v3 ro, v3 rd, uint32_t *rng,
                     orc_stats *st)
@@ -1038,37 +1081,8 @@ static v3 ray_color(const orc_scene *sc, const orc_constant_params *p, v3 ro, v3
             final_color = vmul(final_color, mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f));
             break;
         }
-        uint32_t transform_id = hh.inverted_transform_id + 1u;
         hit_record hit;
-        memset(&hit, 0, sizeof(hit));
-        hit.t = hh.t;
-        hit.p = vadd(ro, vscale(hh.t, rd));
-        hit.material_index = hh.material_index;
-        if (hh.node_type == 1u) {
-            if (st) { st->sphere_hits++; }
-            v3 center = xform(&sc->transforms[16u * transform_id], mk(0.0f, 0.0f, 0.0f), 1.0f);
-            v3 on = vnormalize(vsub(hit.p, center));
-            float theta = orc_acosf(-on.y);
-            float phi = orc_atan2f(-on.z, on.x) + PI_F;
-            hit.uv[0] = phi / (2.0f * PI_F);
-            hit.uv[1] = theta / PI_F;
-            set_face_normal(&hit, rd, on);
-        } else if (hh.node_type == 2u) {
-            if (st) { st->mesh_hits++; }
-            uint32_t tri = hh.tri_id;
-            v3 n0 = vload(&sc->normals[4u * sc->normal_indices[tri]]);
-            v3 n1 = vload(&sc->normals[4u * sc->normal_indices[tri + 1u]]);
-            v3 n2 = vload(&sc->normals[4u * sc->normal_indices[tri + 2u]]);
-            const float *uv0 = &sc->uvs[2u * sc->uv_indices[tri]];
-            const float *uv1 = &sc->uvs[2u * sc->uv_indices[tri + 1u]];
-            const float *uv2 = &sc->uvs[2u * sc->uv_indices[tri + 2u]];
-            float w = 1.0f - hh.uv[0] - hh.uv[1];
-            v3 normal = vadd(vadd(vscale(w, n0), vscale(hh.uv[0], n1)), vscale(hh.uv[1], n2));
-            hit.uv[0] = w * uv0[0] + hh.uv[0] * uv1[0] + hh.uv[1] * uv2[0];
-            hit.uv[1] = w * uv0[1] + hh.uv[0] * uv1[1] + hh.uv[1] * uv2[1];
-            v3 on = vnormalize(xform_normal(&sc->transforms[16u * hh.inverted_transform_id], normal));
-            set_face_normal(&hit, rd, on);
-        }
+        make_hit_record(sc, ro, rd, &hh, &hit, st);
         v3 att, so, sd;
         if (material_scatter(sc, p, rd, &hit, rng, &att, &so, &sd, st)) {
             ro = so;
@@ -1081,6 +1095,67 @@ static v3 ray_color(const orc_scene *sc, const orc_constant_params *p, v3 ro, v3
     }
     return final_color;
 }
+
+/* ---- exported pieces of ray_color (closed-form tests) ---- */
+static void record_out(const hit_record *h, orc_hit_record *o)
+{
+    vstore(o->p, h->p);
+    vstore(o->normal, h->normal);
+    o->uv[0] = h->uv[0];
+    o->uv[1] = h->uv[1];
+    o->material_index = h->material_index;
+    o->t = h->t;
+    o->front_face = h->front_face;
+}
+
+static void record_in(const orc_hit_record *o, hit_record *h)
+{
+    memset(h, 0, sizeof(*h));
+    h->p = vload(o->p);
+    h->normal = vload(o->normal);
+    h->uv[0] = o->uv[0];
+    h->uv[1] = o->uv[1];
+    h->material_index = o->material_index;
+    h->t = o->t;
+    h->front_face = o->front_face;
+}
+
+void orc_make_hit_record(const orc_scene *sc, const float ro[3], const float rd[3], const orc_hit *hit,
+                         orc_hit_record *out)
+{
+    hit_record h;
+    make_hit_record(sc, vload(ro), vload(rd), hit, &h, NULL);
+    record_out(&h, out);
+}
+
+int orc_material_scatter(const orc_scene *sc, const orc_constant_params *p, const float rd[3],
+                         const orc_hit_record *h, uint32_t *rng, float att[3], float origin[3],
+                         float direction[3])
+{
+    hit_record hr;
+    record_in(h, &hr);
+    v3 a = mk(0.0f, 0.0f, 0.0f), so = a, sd = a;
+    int r = material_scatter(sc, p, vload(rd), &hr, rng, &a, &so, &sd, NULL);
+    vstore(att, a);
+    vstore(origin, so);
+    vstore(direction, sd);
+    return r;
+}
+
+void orc_material_emit(const orc_scene *sc, const orc_constant_params *p, const orc_hit_record *h,
+                       float out[3])
+{
+    hit_record hr;
+    record_in(h, &hr);
+    vstore(out, material_emit(sc, p, &hr, NULL));
+}
+
+void orc_sample_texture(const orc_texture *tx, const float uv[2], float out[3])
+{
+    vstore(out, sample_texture(tx, uv));
+}
+
+float orc_half_to_float(uint16_t h) { return half_to_float(h); }
 
 /* pathtracer.wgsl:64-79 (one sample) */
 static v3 sample_pixel(const orc_scene *sc, const orc_constant_params *p, uint32_t x, uint32_t y,

@@ -199,6 +199,34 @@ float orc_render(const orc_scene *sc, const orc_constant_params *params, uint32_
 float orc_render_pixels(const orc_scene *sc, const orc_constant_params *params,
                         uint32_t iterations, const uint32_t *pixels, uint32_t n, float *acc,
                         uint32_t *rng, float *fb, int nthreads, orc_stats *stats);
+/* The pieces of ray_color, exposed for closed-form tests; ray_color runs
+ * exactly these functions. */
+typedef struct orc_hit_record {  /* hitrecord.wgsl:1-8 */
+    float p[3];
+    float normal[3];  /* shading normal, flipped to face the ray */
+    float uv[2];
+    uint32_t material_index;
+    float t;
+    int32_t front_face;
+} orc_hit_record;
+/* The hit record of a closest hit (pathtracer.wgsl:111-144 +
+ * hitrecord.wgsl:10-18): sphere normal from the world-space centre and uv
+ * from acos/atan2; mesh normal and uv interpolated and the normal taken
+ * through transpose(inverse) (transform.wgsl:16-19). */
+void orc_make_hit_record(const orc_scene *sc, const float ro[3], const float rd[3], const orc_hit *hit,
+                         orc_hit_record *out);
+/* material_scatter (material.wgsl:61-77): returns 1 when the path goes on. */
+int orc_material_scatter(const orc_scene *sc, const orc_constant_params *p, const float rd[3],
+                         const orc_hit_record *h, uint32_t *rng, float att[3], float origin[3],
+                         float direction[3]);
+/* material_emit (material.wgsl:79-89). */
+void orc_material_emit(const orc_scene *sc, const orc_constant_params *p, const orc_hit_record *h,
+                       float out[3]);
+/* textureSampleLevel(t, default sampler, uv, 0) (material.wgsl:91-99,
+ * webgpu.zig:812-825). */
+void orc_sample_texture(const orc_texture *tx, const float uv[2], float out[3]);
+float orc_half_to_float(uint16_t h);
+
 /* RNG buffer seeding (buffers.zig:25-30): state = index + seed * pixel_count. */
 void orc_seed_rng(uint32_t *rng, uint32_t pixel_count, uint32_t seed);
 

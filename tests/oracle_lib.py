@@ -35,6 +35,16 @@ class Texture(C.Structure):
                 ("bytes_per_component", u32), ("is_hdr", u32), ("pad", u32), ("data", vp)]
 
 
+class HitRecord(C.Structure):
+    _fields_ = [("p", f32 * 3), ("normal", f32 * 3), ("uv", f32 * 2), ("material_index", u32),
+                ("t", f32), ("front_face", C.c_int32)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("hit", u32), ("t", f32), ("material_index", u32), ("node_type", u32),
+                ("inverted_transform_id", u32), ("tri_id", u32), ("uv", f32 * 2)]
+
+
 class OScene(C.Structure):
     _fields_ = [("tlas", vp), ("tlas_len", u32), ("blas", vp), ("blas_len", u32),
                 ("normals", vp), ("normals_len", u32), ("normal_indices", vp),
@@ -123,6 +133,12 @@ def lib():
             "orc_render_pixels": (f32, [P(OScene), P(ConstantParams), u32, vp, u32, vp, vp, vp,
                                         C.c_int, P(Stats)]),
             "orc_seed_rng": (None, [vp, u32, u32]),
+            "orc_make_hit_record": (None, [P(OScene), P(f32), P(f32), P(Hit), P(HitRecord)]),
+            "orc_material_scatter": (C.c_int, [P(OScene), P(ConstantParams), P(f32), P(HitRecord),
+                                               P(u32), P(f32), P(f32), P(f32)]),
+            "orc_material_emit": (None, [P(OScene), P(ConstantParams), P(HitRecord), P(f32)]),
+            "orc_sample_texture": (None, [P(Texture), P(f32), P(f32)]),
+            "orc_half_to_float": (f32, [C.c_uint16]),
             "orc_mat_inverse": (C.c_int, [P(f32), P(f32)]),
             "orc_mat_mul": (None, [P(f32), P(f32), P(f32)]),
             "orc_mat_from_quat": (None, [P(f32), P(f32)]),
