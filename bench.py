@@ -15,21 +15,32 @@ parameters for the scene class; its JSON cache ships with the package
 trial renders. Tuning never changes the image.
 
 Also reported:
-  roofline     algorithmic bytes per render (the reference algorithm's node,
-               transform, material and attribute fetches, counted on the same
-               paths by the instrumented CPU oracle in the cpu_baseline leg)
-               divided by the render kernel's average duration (HIP events on
-               the stream the kernel runs on), against 8 TB/s of HBM3E;
-               `traffic` is the HBM bytes per launch that rocprofv3's
-               FETCH_SIZE and WRITE_SIZE counters measured for this command,
-               read from the committed profile summary. The fraction can pass
-               1 because those bytes are served from on-chip caches; what
-               bounds the kernel is vector-instruction issue, whose share of
-               SIMD cycles (SQ counters, committed summary) is reported as
-               `valu_issue_frac`.
+  roofline     the bound that limits the render kernel: vector-instruction
+               issue. `achieved` = the lanes that did VALU work per launch
+               (SQ_THREAD_CYCLES_VALU, from the committed rocprofv3 counter
+               summary of this workload and launch) divided by the kernel's
+               average duration (HIP events on the stream it runs on);
+               `peak` = 39.3e12 lane-operations/s, MI355X's published FP32
+               vector peak (157.3 TFLOP/s) without its factor 2 for FMA and 2
+               for packed FP32 (1,024 SIMDs x 16 lanes x 2.4 GHz). Also:
+               `lane_occupancy` (active lanes per VALU instruction / 64),
+               `valu_issue_frac` (VALU issue share of SIMD cycles), and `hbm`:
+               HBM bytes per launch from FETCH_SIZE/WRITE_SIZE (scaled by a
+               plain copy kernel) over the live kernel time against 8 TB/s,
+               beside the reference algorithm's bytes per second (node,
+               transform, material and attribute fetches counted on the same
+               paths by the instrumented CPU oracle), which on-chip caches
+               serve and which therefore exceed the HBM peak.
   cpu_baseline the oracle (a C restatement of the WGSL megakernel, OpenMP
                over pixels, serial samples per pixel) timed on the host on a
-               bounded sample of the same workload.
+               bounded sample of the same workload, on every CPU the process
+               may use up to the box's allotment for one GPU (16 threads);
+               nproc, os.cpu_count() and the CPU model are reported.
+  workloads    the other configurations timed the same way (median kernel
+               time of three renders after a warm-up, shipped tuning): the
+               Cornell box (C1), the instanced icospheres (C3), and the batch
+               of 64 instanced-icosphere scenes in one graph-replayed launch
+               (C4).
   equal_time_rmse
                plain path tracing vs the neural radiance cache on the
                instanced-icosphere scene (1,024 instances of a 327,680-triangle
@@ -66,8 +77,16 @@ WORKLOADS = {
     "C2": ("random_spheres_1920x1080_64spp_depth16", "random-spheres 1920x1080 64spp depth 16"),
     "C3": ("instanced_icospheres_1920x1080_32spp_depth8", "1,024 instanced icospheres 1920x1080 32spp depth 8"),
 }
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "render_kernel_random_spheres_1080p_64spp_pmc.md")
-ISSUE_PROFILE = os.path.join(ROOT, "profiles", "render_kernel_random_spheres_1080p_64spp_sq.md")
+# committed rocprofv3 counter summaries of the render kernel per workload:
+# HBM traffic (FETCH_SIZE / WRITE_SIZE) and instruction counters (SQ_*)
+PROFILES = {
+    "C2": ("render_kernel_random_spheres_1080p_64spp_pmc.md", "render_kernel_random_spheres_1080p_64spp_sq.md"),
+    "C3": ("render_kernel_instanced_icospheres_1080p_32spp_pmc.md",
+           "render_kernel_instanced_icospheres_1080p_32spp_sq.md"),
+}
+# vector lanes the SIMDs can drive per second: 157.3 TFLOP/s FP32 (FMA = 2,
+# packed FP32 = 2) / 4 = 1,024 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_LANE_OPS = 157.3e12 / 4
 
 
 def profiled_figure(path, name, kind=float):
@@ -81,10 +100,27 @@ def profiled_figure(path, name, kind=float):
     return kind(m.group(1)) if m else None
 
 
-def profiled_traffic():
-    """HBM bytes per render launch (FETCH_SIZE + WRITE_SIZE, scaled by a
-    plain copy kernel), or None."""
-    return profiled_figure(TRAFFIC_PROFILE, "traffic_bytes_per_launch", int)
+def profile_path(workload, kind):
+    names = PROFILES.get(workload)
+    return os.path.join(ROOT, "profiles", names[kind]) if names else ""
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_threads():
+    """The CPUs this process may run on, at most the box's allotment for one
+    GPU (OMP_NUM_THREADS, 16 by default)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(avail, cap)), avail
 
 
 def algorithmic_bytes(stats):
@@ -106,7 +142,7 @@ def cpu_baseline(spec, arrays, params, seconds=12.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     O.build()
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_threads()
     n_px = spec.width * spec.height
     sc = O.OracleScene(arrays)
     p = O.ConstantParams.from_buffer_copy(bytes(params))
@@ -134,17 +170,6 @@ def rmse(img, ref):
     d = np.asarray(img, np.float64)[:, :3] - np.asarray(ref, np.float64)[:, :3]
     ok = np.all(np.isfinite(d), axis=1)
     return float(np.sqrt(np.mean(d[ok] ** 2)))
-
-
-def orbit_lookfrom(spec, frame, frames):
-    """Camera position of a frame of the seeded path: a quarter orbit around
-    the vertical axis through the scene centre, from the scene's own
-    camera position."""
-    lookfrom = np.array(spec.camera[0], np.float64)
-    a = 0.5 * np.pi * frame / max(frames - 1, 1)
-    c, s = np.cos(a), np.sin(a)
-    x, y, z = lookfrom
-    return (float(c * x + s * z), float(y), float(-s * x + c * z))
 
 
 def relmse(img, ref):
@@ -176,7 +201,7 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
     lookat, vup = spec.camera[1], spec.camera[2]
 
     def params_at(frame):
-        scene.set_look_at(orbit_lookfrom(spec, frame, frames), lookat, vup)
+        scene.set_look_at(W.orbit_lookfrom(spec, frame, frames), lookat, vup)
         p = zo._native.ConstantParams()
         p.camera = scene.camera()
         p.depth, p.width, p.height = spec.depth, spec.width, spec.height
@@ -220,12 +245,15 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
     pt = zo.Target(ds, spec.width, spec.height, 7)
     for spp in (1, 2, 3, 4, 6, 8, 12, 16, 24, 32):
         pt.autotune(p, spp, cache_path=zo.TUNING_CACHE)  # the plain tracer's best launch for this spp
-        pt.set_state(np.zeros((n, 4), np.float32), np.arange(n, dtype=np.uint32) + np.uint32(7 * n))
-        pt.render(p, spp)
+        ms = []
+        for _ in range(3):  # median of three renders of the same samples
+            pt.set_state(np.zeros((n, 4), np.float32), np.arange(n, dtype=np.uint32) + np.uint32(7 * n))
+            pt.render(p, spp)
+            ms.append(pt.last_render_ms())
         img = pt.accumulation() / spp
-        plain.append({"spp": spp, "ms": pt.last_render_ms(), "rmse": rmse(img, reference),
+        plain.append({"spp": spp, "ms": float(np.median(ms)), "rmse": rmse(img, reference),
                       "relmse": relmse(img, reference)})
-        if plain[-1]["rmse"] <= target and plain[-1]["ms"] > slowest:
+        if spp >= 8 and plain[-1]["rmse"] <= target and plain[-1]["ms"] > slowest:
             break
     out = {"scene": "instanced_icospheres_1920x1080_depth8", "frames": frames, "reference_spp": ref_spp}
     for name, r in runs.items():
@@ -237,8 +265,81 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
                                              f"longer than the cache's frame")
         r["plain_to_match_rmse"] = match
         r["plain_time_over_nrc_time"] = match["ms"] / r["ms_per_frame"] if match else None
+        fit = equal_error_plain(plain, r["rmse"])
+        r["plain_equal_error_interpolated"] = fit
+        r["plain_time_over_nrc_time_interpolated"] = fit["ms"] / r["ms_per_frame"] if fit else None
         out[name] = r
     out["plain_curve"] = plain
+    return out
+
+
+def equal_error_plain(plain, target, max_spp=8):
+    """The plain tracer's samples per pixel and time at exactly the RMSE
+    `target`, interpolated instead of rounded up to a whole sample count:
+    MSE(spp) = a + b / spp (independent samples: variance falls as 1/spp; a
+    holds the reference's own noise) and time(spp) = c + d * spp, both fitted
+    by least squares on the measured points up to `max_spp`."""
+    pts = [q for q in plain if q["spp"] <= max_spp]
+    if len(pts) < 3:
+        return None
+    spp = np.array([q["spp"] for q in pts], np.float64)
+    mse = np.array([q["rmse"] for q in pts], np.float64) ** 2
+    ms = np.array([q["ms"] for q in pts], np.float64)
+    (a, b), *_ = np.linalg.lstsq(np.stack([np.ones_like(spp), 1 / spp], 1), mse, rcond=None)
+    (c, d), *_ = np.linalg.lstsq(np.stack([np.ones_like(spp), spp], 1), ms, rcond=None)
+    if target ** 2 <= a or b <= 0:
+        return None
+    s_star = b / (target ** 2 - a)
+    return {"spp": float(s_star), "ms": float(c + d * s_star), "fit_points": [int(x) for x in spp],
+            "mse_fit": [float(a), float(b)], "ms_fit": [float(c), float(d)]}
+
+
+def time_target_workload(dev, key, steps=3):
+    """Median render-kernel time of one of the single-image configurations
+    with its shipped tuning (three renders after a warm-up)."""
+    spec = W.CONFIGS[key]()
+    ds = zo.DeviceScene(dev, zo.Bvh(W.build_scene(spec)))
+    t = zo.Target(ds, spec.width, spec.height, 0)
+    params = W.spec_params(spec)
+    t.autotune(params, spec.spp, cache_path=zo.TUNING_CACHE)
+    t.render(params, spec.spp)
+    ms, wall = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        t.render(params, spec.spp)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        ms.append(t.last_render_ms())
+    med = float(np.median(ms))
+    out = {"workload": WORKLOADS[key][0], "samples": spec.samples, "kernel_ms_median": med,
+           "wall_ms_median": float(np.median(wall)), "samples_per_s": spec.samples / (med * 1e-3),
+           "launch_tuning": t.tuning()}
+    t.destroy()
+    ds.destroy()
+    return out
+
+
+def time_batch_workload(dev, steps=3):
+    """The same for C4: 64 instanced-icosphere scenes in one graph-replayed
+    persistent launch (workloads.batch_scenes)."""
+    specs = W.batch_scenes()
+    arrays = [zo.Bvh(W.build_scene(s)) for s in specs]
+    batch = zo.Batch(dev, [(a, s.width, s.height, s.seed) for a, s in zip(arrays, specs)])
+    params = [W.spec_params(s) for s in specs]
+    spp = [s.spp for s in specs]
+    batch.autotune(params, spp, cache_path=zo.TUNING_CACHE)
+    batch.render(params, spp)
+    ms, wall = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        batch.render(params, spp)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        ms.append(batch.last_render_ms())
+    samples = sum(s.samples for s in specs)
+    med = float(np.median(ms))
+    out = {"workload": "batch_64_instanced_icosphere_scenes_128-1024px_1-64spp_depth8", "images": len(specs),
+           "samples": samples, "kernel_ms_median": med, "wall_ms_median": float(np.median(wall)),
+           "samples_per_s": samples / (med * 1e-3), "launch_tuning": batch.tuning()}
+    batch.destroy()
     return out
 
 
@@ -262,6 +363,8 @@ def main():
                     help="autotuner cache (JSON); a scene class missing from it is measured")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch tuning")
     ap.add_argument("--no-nrc", action="store_true", help="skip the equal-time RMSE leg")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip timing the other configurations (C1, C3, C4)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="C2",
                     help="the timed scene (C2, random spheres, is the headline; C1 and C3 print the "
                          "same line for the Cornell box and the instanced icospheres)")
@@ -340,25 +443,44 @@ def main():
     }
     if calib:
         out["calibration_copy"] = calib
+    # the bound: vector issue at partial lane occupancy (counters from the
+    # committed summary of this workload's launch, time measured live)
+    sq_path, pmc_path = profile_path(args.workload, 1), profile_path(args.workload, 0)
+    lane_ops = profiled_figure(sq_path, "valu_lane_ops_per_launch")
+    traffic = profiled_figure(pmc_path, "traffic_bytes_per_launch", int)
+    roof = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_LANE_OPS / 1e12, "unit": "Tlane-op/s",
+            "frac": None, "traffic": traffic,
+            "lane_occupancy": profiled_figure(sq_path, "lane_occupancy"),
+            "valu_issue_frac": profiled_figure(sq_path, "valu_issue_fraction"),
+            "counters": os.path.relpath(sq_path, ROOT) if lane_ops else None}
+    if lane_ops:
+        roof["achieved"] = lane_ops / avg_kernel_s / 1e12
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    hbm = {"peak_GBps": HBM_PEAK_GBS}
+    if traffic:
+        hbm["measured_GBps"] = traffic / avg_kernel_s / 1e9
+        hbm["frac"] = hbm["measured_GBps"] / HBM_PEAK_GBS
+    roof["hbm"] = hbm
+    out["roofline"] = roof
     if not args.no_cpu_baseline:
         cb = cpu_baseline(spec, arrays, params, args.cpu_seconds)
         bytes_per_launch = cb["bytes_per_sample"] * spec.samples + 16 * spec.width * spec.height
-        achieved = bytes_per_launch / avg_kernel_s / 1e9
-        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": profiled_traffic() if args.workload == "C2" else None,
-                           "valu_issue_frac": (profiled_figure(ISSUE_PROFILE, "valu_issue_fraction")
-                                               if args.workload == "C2" else None),
-                           "algorithmic_bytes_per_sample": cb["bytes_per_sample"],
-                           "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
-                           "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"],
-                           "rays_per_s": value * cb["stats"]["rays"] / cb["stats"]["samples"]}
+        hbm["algorithmic_GBps"] = bytes_per_launch / avg_kernel_s / 1e9
+        roof.update({"algorithmic_bytes_per_sample": cb["bytes_per_sample"],
+                     "rays_per_sample": cb["stats"]["rays"] / cb["stats"]["samples"],
+                     "node_visits_per_ray": cb["stats"]["node_visits"] / cb["stats"]["rays"],
+                     "rays_per_s": value * cb["stats"]["rays"] / cb["stats"]["samples"]})
+        threads, avail = host_threads()
         out["cpu_baseline"] = {"value": cb["samples_per_s"], "unit": "samples/s",
                                "cores": cb["threads"], "kind": "port",
                                "sample": f"{cb['pixels']} evenly spaced pixels of the same image "
                                          f"at {spec.spp} spp ({cb['pixels'] * spec.spp} samples, "
-                                         f"{cb['seconds']:.1f} s) on {cb['threads']} host threads"}
+                                         f"{cb['seconds']:.1f} s) on {cb['threads']} host threads",
+                               "nproc": avail, "os_cpu_count": os.cpu_count(), "cpu_model": cpu_model()}
         out["gpu_over_cpu"] = value / cb["samples_per_s"]
+    if not args.no_workloads and args.workload == "C2":
+        out["workloads"] = {k: time_target_workload(dev, k) for k in ("C1", "C3")}
+        out["workloads"]["C4"] = time_batch_workload(dev)
     if not args.no_nrc:
         out["equal_time_rmse"] = equal_time_rmse(dev)
     print(json.dumps(out), flush=True)

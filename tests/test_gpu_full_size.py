@@ -2,6 +2,8 @@
 Cornell 256x256 16 spp; instanced icospheres 1920x1080 32 spp): properties
 that do not need a full CPU reference, plus bit-exact parity with the oracle
 on a seeded subset of pixels rendered at full spp."""
+import json
+
 import numpy as np
 import pytest
 
@@ -23,15 +25,34 @@ def check_subset(prod, spec, acc_g, rng_g, pixels):
     assert rng.tobytes() == rng_g[pixels].tobytes()
 
 
+def cached_entry(tuning, ms):
+    """The shipped tuning-cache entry a cache hit returned (bench.py applies
+    the same one)."""
+    entries = json.load(open(zo.TUNING_CACHE)).values()
+    hits = [e for e in entries if e["ms"] == pytest.approx(ms, rel=1e-6)
+            and all(e[k] == v for k, v in tuning.items() if k in e)]
+    assert len(hits) == 1, (tuning, ms)
+    return hits[0]
+
+
 @pytest.mark.parametrize("name", ["C2", "C3"])
 def test_full_size_render(device, name):
-    """BVH4 (default) render at full size; the spp/2 + spp/2 render uses the
-    BVH2 and a different schedule, and must give the same bits."""
+    """The launch bench.py times (the shipped tuning-cache entry of the
+    scene class: split pixels handed between lanes, several nodes per round)
+    at full size; the spp/2 + spp/2 render uses a different schedule without
+    pixel jobs and must give the same bits, and a seeded subset must equal
+    the oracle."""
     spec = W.random_spheres() if name == "C2" else W.instanced_icospheres()
     print(f"[full] {name} {spec.width}x{spec.height} {spec.spp} spp depth {spec.depth}", flush=True)
     prod = zo.Bvh(W.build_scene(spec))
     ds = zo.DeviceScene(device, prod)
     a = zo.Target(ds, spec.width, spec.height)
+    tuning, ms = a.autotune(zo_params(spec), spec.spp, cache_path=zo.TUNING_CACHE)
+    entry = cached_entry(tuning, ms)
+    print(f"[full] {name} timed launch tuning {entry}", flush=True)
+    assert a.tuning() == tuning
+    if name == "C2":  # the headline launch hands split pixels from job to job
+        assert tuning["split_percent"] > 0 and 0 < tuning["samples_per_job"] < spec.spp
     a.render(zo_params(spec), spec.spp)
     acc_a, rng_a = a.accumulation(), a.rng_state()
     print(f"[full] {name} render {a.last_render_ms():.1f} ms", flush=True)

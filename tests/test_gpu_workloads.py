@@ -1,0 +1,152 @@
+"""GPU: the batched (C4) and radiance-cache (C5) workloads at their full
+sizes.
+
+C4: 64 seeded instanced-icosphere scenes (10^3 - 2*10^6 triangles, 128^2 -
+1024^2, 1 - 64 spp, depth 8) in one persistent, graph-replayed launch with the
+shipped tuning-cache entry (the launch the batch benchmark times). Every image
+accumulates exactly its spp, stays finite, and equals the oracle bit for bit
+on 256 seeded pixels.
+
+C5: the neural radiance cache on the 1,024-instance icosphere scene at
+1920x1080, 1 spp per frame along the seeded orbit, 2^19-entry tables and
+65,536 training paths per frame (the workload's schedule): the training loss
+stays finite and falls, frames render and train like separate calls, and the
+cache's 1-spp image is closer to a high-spp plain reference than a plain
+1-spp image."""
+import json
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import zig_ornament_amd as zo
+from helpers import same_bits, zo_params
+from zig_ornament_amd import workloads as W
+from zig_ornament_amd.nrc import Nrc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c4_batch_at_workload_size(device):
+    specs = W.batch_scenes()
+    assert len(specs) == 64
+    arrays = [zo.Bvh(W.build_scene(s)) for s in specs]
+    batch = zo.Batch(device, [(a, s.width, s.height, s.seed) for a, s in zip(arrays, specs)])
+    params = [zo_params(s) for s in specs]
+    iters = [s.spp for s in specs]
+    tuning, ms = batch.autotune(params, iters, cache_path=zo.TUNING_CACHE)
+    entries = json.load(open(zo.TUNING_CACHE)).values()
+    assert any(e["ms"] == pytest.approx(ms, rel=1e-6) for e in entries), "C4 tuning not from the cache"
+    print(f"[c4] {len(specs)} images, {sum(s.samples for s in specs):,} samples, tuning {tuning}", flush=True)
+    batch.render(params, iters)
+    print(f"[c4] batch render {batch.last_render_ms():.1f} ms", flush=True)
+    checked = 0
+    for i, (s, a) in enumerate(zip(specs, arrays)):
+        n = s.width * s.height
+        acc = batch.accumulation(i)
+        rng = batch.rng_state(i)
+        assert np.all(acc[:, 3] == s.spp), s.name
+        assert np.all(np.isfinite(acc)), s.name  # lambertian only: no total-internal-reflection NaN
+        pixels = np.random.default_rng(100 + i).choice(n, min(256, n), replace=False).astype(np.uint32)
+        o_acc = np.zeros((len(pixels), 4), np.float32)
+        o_rng = (pixels + np.uint32(s.seed * n)).astype(np.uint32)
+        p = O.ConstantParams.from_buffer_copy(bytes(params[i]))
+        O.OracleScene(a).render_pixels(p, s.spp, pixels, o_acc, o_rng)
+        assert same_bits(o_acc, acc[pixels]), s.name
+        assert o_rng.tobytes() == rng[pixels].tobytes(), s.name
+        fb = batch.framebuffer(i)
+        assert np.all((fb >= 0.0) & (fb <= 1.0))
+        checked += len(pixels)
+    print(f"[c4] {checked} pixels bit-exact vs the oracle", flush=True)
+    batch.destroy()
+
+
+def _rmse(img, ref):
+    d = np.asarray(img, np.float64)[:, :3] - np.asarray(ref, np.float64)[:, :3]
+    return float(np.sqrt(np.mean(d ** 2)))
+
+
+@pytest.fixture(scope="module")
+def c5(device):
+    spec = W.instanced_icospheres(spp=1)
+    scene = W.build_scene(spec)
+    ds = zo.DeviceScene(device, zo.Bvh(scene))
+    lookat, vup = spec.camera[1], spec.camera[2]
+
+    def params_at(frame, frames):
+        scene.set_look_at(W.orbit_lookfrom(spec, frame, frames), lookat, vup)
+        p = zo_params(spec)
+        p.camera = scene.camera()
+        return p
+
+    return spec, ds, params_at
+
+
+C5_CACHE = dict(log2_table=19, train_paths=65536, train_batch=65536, terminate_bounce=0)
+
+
+def test_c5_cache_at_workload_size(c5):
+    spec, ds, params_at = c5
+    frames = 16
+    nrc = Nrc(ds, **C5_CACHE)
+    t = zo.Target(ds, spec.width, spec.height, 7)
+    losses, times = [], []
+    for f in range(frames):
+        ms, loss = nrc.frame(t, params_at(f, frames), 1, f, steps=1, want_loss=True)
+        losses.append(loss)
+        times.append(ms)
+    print(f"[c5] losses {['%.4f' % x for x in losses]}", flush=True)
+    print(f"[c5] frame ms median {np.median(times):.2f}", flush=True)
+    assert all(np.isfinite(losses))
+    assert np.median(losses[-4:]) < np.median(losses[:3])
+    mlp, grid = nrc.get_params()
+    assert np.all(np.isfinite(mlp)) and np.all(np.isfinite(grid))
+    img = t.accumulation()
+    assert np.all(img[:, 3] == 1.0) and np.all(np.isfinite(img))
+    # the last view: cache vs plain at 1 spp against a 1,024-spp plain reference
+    p = params_at(frames - 1, frames)
+    ref = zo.Target(ds, spec.width, spec.height, 99)
+    for k in range(4):
+        q = zo_params(spec, current_iteration=float(256 * k))
+        q.camera = p.camera
+        ref.render(q, 256)
+    reference = ref.accumulation() / 1024.0
+    plain = zo.Target(ds, spec.width, spec.height, 7)
+    plain.render(p, 1)
+    e_nrc, e_plain = _rmse(img, reference), _rmse(plain.accumulation(), reference)
+    print(f"[c5] last frame 1-spp RMSE: cache {e_nrc:.4f}, plain {e_plain:.4f}", flush=True)
+    assert e_nrc < 0.9 * e_plain
+    nrc.destroy()
+
+
+def test_c5_frames_equal_separate_calls_at_workload_size(c5):
+    """Two frames of orn_nrc_frame against render + train on a second cache
+    fed the same calls, at the workload's sizes (Adam epsilon 1e-3 so that
+    parameters whose gradient sits at rounding level move by lr * g / eps, not
+    by +-lr in the direction of their rounding noise)."""
+    spec, ds, params_at = c5
+    cfg = dict(C5_CACHE, epsilon=1e-3)
+    a, b = Nrc(ds, **cfg), Nrc(ds, **cfg)
+    ta = zo.Target(ds, spec.width, spec.height, 3)
+    tb = zo.Target(ds, spec.width, spec.height, 3)
+    p0, p1 = params_at(0, 120), params_at(1, 120)
+    _, la = a.frame(ta, p0, 1, 0, steps=1, want_loss=True)
+    b.render(tb, p0, 1)
+    assert same_bits(ta.accumulation(), tb.accumulation())
+    lb = b.train(p0, 0, steps=1)
+    assert abs(la - lb) <= 1e-5 * lb
+    a.frame(ta, p1, 1, 1, steps=1)
+    b.render(tb, p1, 1)
+    ia, ib = ta.accumulation(), tb.accumulation()
+    d = np.abs(ia - ib)
+    print(f"[c5] frame-1 image: max |diff| {d.max():.3e}, mean {d.mean():.3e}", flush=True)
+    assert d.max() <= 1e-3 * np.abs(ib).max() and d.mean() <= 1e-6 * np.abs(ib).mean()
+    b.train(p1, 1, steps=1)
+    ma, ga = a.get_params()
+    mb, gb = b.get_params()
+    assert np.array_equal(ma, mb)
+    rel = np.linalg.norm(ga - gb) / np.linalg.norm(gb)
+    print(f"[c5] grid difference after 2 steps: {rel:.2e} (relative norm)", flush=True)
+    assert rel <= 1e-6
+    a.destroy()
+    b.destroy()

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
             store_coherent(acc_rsrc, pix, acc);
             store_coherent(rng_buf + pix, rng);
             if (release_tag != 0u) {  // a release is still pending: write it now
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 store_coherent(stage + release_pix, release_tag);
             }
             release_pix = pix;
@@ -287,6 +287,10 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
             rng = rng_buf[pix];
             acc = acc_buf[pix];
         } else {  // state handed on within this launch
+            // job_ready() read the predecessor's stage word (relaxed); this
+            // acquire fence pairs with the release fence before that word's
+            // store, so the state stores below it are visible here
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             rng = load_coherent(rng_buf + pix);
             acc = load_coherent(acc_rsrc, pix);
         }
@@ -299,12 +303,14 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         start_sample();
         next_traversal(false, thr);
     };
-    // Stage words of finished jobs are written once the job's state stores
-    // have completed: a wait on all of the wave's memory operations, which by
-    // the next shading round have long been done, so the wait is short.
+    // Stage words of finished jobs are written after a release fence (agent
+    // scope) that orders the job's state stores before them: the HIP memory
+    // model's fence-to-fence synchronisation with the acquire fence in
+    // begin_job. One fence per wave serves every lane's release; by the next
+    // shading round the stores have long completed, so it waits briefly.
     auto flush_releases = [&]() {
         if (lanes_where(release_tag != 0u) != 0u) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (release_tag != 0u) {
                 store_coherent(stage + release_pix, release_tag);
                 release_tag = 0u;

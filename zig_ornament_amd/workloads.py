@@ -290,6 +290,17 @@ def instanced_icospheres(width=1920, height=1080, spp=32, depth=8, levels=7, ins
     return spec
 
 
+def orbit_lookfrom(spec, frame, frames):
+    """C5's seeded camera path: frame `frame` of `frames` on a quarter orbit
+    around the vertical axis through the scene centre, starting at the
+    scene's own camera position (look-at point and up vector unchanged)."""
+    lookfrom = np.array(spec.camera[0], np.float64)
+    a = 0.5 * np.pi * frame / max(frames - 1, 1)
+    c, s = np.cos(a), np.sin(a)
+    x, y, z = lookfrom
+    return (float(c * x + s * z), float(y), float(-s * x + c * z))
+
+
 def instanced_scene(triangles, side, spp, depth=8, seed=5, texture_size=128):
     """One image of the batch workload: icospheres instanced until the scene
     holds about `triangles` triangles (level 3, 1,280 triangles each, below
