@@ -9,7 +9,7 @@ pixels, and equals the oracle bit for bit
 on 256 seeded pixels and on every pixel with a non-finite sample.
 
 C5: the neural radiance cache on the 1,024-instance icosphere scene at
-1920x1080, 1 spp per frame along the seeded orbit, 2^19-entry tables and
+1920x1080, 1 spp per frame along the seeded 120-frame orbit, 2^19-entry tables and
 65,536 training paths per frame (the workload's schedule): the training loss
 stays finite and falls, frames render and train like separate calls, and the
 cache's 1-spp image is closer to a high-spp plain reference than a plain
@@ -94,8 +94,9 @@ C5_CACHE = dict(log2_table=19, train_paths=65536, train_batch=65536, terminate_b
 
 
 def test_c5_cache_at_workload_size(c5):
+    """All 120 frames of the seeded camera path, as the workload runs them."""
     spec, ds, params_at = c5
-    frames = 16
+    frames = 120
     nrc = Nrc(ds, **C5_CACHE)
     t = zo.Target(ds, spec.width, spec.height, 7)
     losses, times = [], []
@@ -103,10 +104,10 @@ def test_c5_cache_at_workload_size(c5):
         ms, loss = nrc.frame(t, params_at(f, frames), 1, f, steps=1, want_loss=True)
         losses.append(loss)
         times.append(ms)
-    print(f"[c5] losses {['%.4f' % x for x in losses]}", flush=True)
+    print(f"[c5] losses {['%.4f' % x for x in losses[:8]]} ... {['%.4f' % x for x in losses[-8:]]}", flush=True)
     print(f"[c5] frame ms median {np.median(times):.2f}", flush=True)
     assert all(np.isfinite(losses))
-    assert np.median(losses[-4:]) < np.median(losses[:3])
+    assert np.median(losses[-10:]) < 0.5 * np.median(losses[:3])
     mlp, grid = nrc.get_params()
     assert np.all(np.isfinite(mlp)) and np.all(np.isfinite(grid))
     img = t.accumulation()
@@ -123,7 +124,7 @@ def test_c5_cache_at_workload_size(c5):
     plain.render(p, 1)
     e_nrc, e_plain = _rmse(img, reference), _rmse(plain.accumulation(), reference)
     print(f"[c5] last frame 1-spp RMSE: cache {e_nrc:.4f}, plain {e_plain:.4f}", flush=True)
-    assert e_nrc < 0.9 * e_plain
+    assert e_nrc < 0.85 * e_plain  # 0.74 in the benchmark's 120-frame runs (4,096-spp reference)
     nrc.destroy()
 
 
