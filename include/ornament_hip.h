@@ -45,6 +45,7 @@ typedef int orn_status;
 #define ORN_ERROR_SIZE_MISMATCH 9007       /* destination smaller than the target */
 #define ORN_ERROR_IO 9008                  /* a file could not be opened, read or written */
 #define ORN_ERROR_INVALID_FILE 9009        /* a scene or image file is malformed */
+#define ORN_ERROR_SCRATCH_TOO_SMALL 9010   /* image larger than orn_render_noalloc's scratch */
 
 /* Human-readable name of a status ("InvalidValue", "EmptyScene", ...). */
 const char *orn_status_string(orn_status status);
@@ -261,6 +262,28 @@ typedef struct orn_tuning {
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);
+
+/* A render that allocates nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It does what
+ * orn_render_async does for a target, on image state the caller owns, all
+ * device pointers on the scene's device in the layouts orn_get_* return:
+ * `accumulation` (width * height float4, read and written), `rng`
+ * (width * height uint32, read and written) and `framebuffer`
+ * (width * height float4, written); width and height come from `params`.
+ * The launch's scratch (the job-queue counters and one stage word per pixel
+ * for split pixels) is a library-owned device array, one per device, sized
+ * for ORN_RENDER_SCRATCH_PIXELS pixels (8 MiB) and held until the process
+ * ends; larger images return ORN_ERROR_SCRATCH_TOO_SMALL before anything is
+ * enqueued. `tuning` may be NULL (the default tuning). `stream` is a
+ * hipStream_t of the scene's device, or NULL for the device context's own
+ * stream. The call returns once the work is enqueued. Every call on a device
+ * uses the same scratch: calls whose work overlaps in time on one device (on
+ * different streams) race; calls on one stream are ordered and do not. */
+#define ORN_RENDER_SCRATCH_PIXELS (1u << 21)
+orn_status orn_render_noalloc(orn_device_scene *scene, const orn_constant_params *params, uint32_t iterations,
+                              const orn_tuning *tuning, float *accumulation, uint32_t *rng, float *framebuffer,
+                              void *stream);
 
 /* GPU time of the most recent render's kernels (HIP events on the target's
  * stream), in milliseconds. */

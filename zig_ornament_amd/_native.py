@@ -125,6 +125,7 @@ DEVICE_API = [
     ("orn_render", st, [vp, P(ConstantParams), u32]),
     ("orn_render_async", st, [vp, P(ConstantParams), u32]),
     ("orn_target_wait", st, [vp]),
+    ("orn_render_noalloc", st, [vp, P(ConstantParams), u32, P(Tuning), vp, vp, vp, vp]),
     ("orn_target_last_render_ms", st, [vp, fp]),
     ("orn_target_set_tuning", st, [vp, P(Tuning)]),
     ("orn_target_get_tuning", st, [vp, P(Tuning)]),

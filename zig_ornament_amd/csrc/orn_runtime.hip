@@ -22,6 +22,7 @@
 #include <cstring>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -911,6 +912,36 @@ hipError_t launch_render(uint32_t features, uint32_t width, uint32_t per_cu, uin
 
 }  // namespace
 
+// Scratch of orn_render_noalloc: the band queue counters, then one stage
+// word per pixel. Each device holds its own copy for the life of the
+// process. Host code reaches it only through the address
+// hipGetSymbolAddress returns for the device (kept in g_scratch); the
+// array's name is not a device address on the host.
+constexpr uint32_t kScratchQueueWords = 64;
+static __device__ __align__(256) uint32_t scratch[kScratchQueueWords + ORN_RENDER_SCRATCH_PIXELS];
+
+struct ScratchSlot {
+    uint32_t *ptr;   // this device's scratch (device address)
+    uint32_t epoch;  // tag of the latest orn_render_noalloc launch on the device
+};
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static ScratchSlot g_scratch[kMaxDevices];
+
+// Called with `device` current (orn_device_open), once per device.
+static hipError_t init_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device].ptr) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device].ptr = static_cast<uint32_t *>(p);
+    g_scratch[device].epoch = 0;  // the array starts zeroed: no stage word carries a tag
+    return hipSuccess;
+}
+
 extern "C" {
 
 const char *orn_status_string(orn_status s)
@@ -926,6 +957,7 @@ const char *orn_status_string(orn_status s)
     case ORN_ERROR_SIZE_MISMATCH: return "SizeMismatch";
     case ORN_ERROR_IO: return "IoError";
     case ORN_ERROR_INVALID_FILE: return "InvalidFile";
+    case ORN_ERROR_SCRATCH_TOO_SMALL: return "ScratchTooSmall";
     default: return hipGetErrorName(static_cast<hipError_t>(s));
     }
 }
@@ -959,6 +991,10 @@ orn_status orn_device_open(int device, orn_ctx **out)
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
+        delete c;
+        return from_hip(e);
+    }
+    if ((e = init_scratch(device)) != hipSuccess) {
         delete c;
         return from_hip(e);
     }
@@ -1247,6 +1283,47 @@ orn_status orn_target_destroy(orn_target *t)
     return ORN_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The render launch of one image on buffers the caller provides: the
+// queues are reset and the kernel enqueued on `st`. `epoch` tags this
+// launch's stage words (unique among the launches that used `stage` since it
+// was last cleared). Allocates nothing.
+hipError_t enqueue_render(orn_device_scene *s, const orn_constant_params *params, uint32_t iterations,
+                          const orn_tuning &tuning, float4 *acc, uint32_t *rng, float4 *fb, uint32_t *stage,
+                          uint32_t *queues, uint32_t epoch, hipStream_t st)
+{
+    KernelParams kp{};
+    kp.p = *params;
+    kp.iterations = iterations;
+    kp.tiles_x = blocks_for(params->width, kTile);
+    kp.tiles = kp.tiles_x * blocks_for(params->height, kTile);
+    kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
+    kp.regroup = tuning.regroup_lanes;
+    kp.leaf_regroup = tuning.leaf_regroup_lanes;
+    kp.node_steps = tuning.node_steps;
+    const uint32_t width = tuning.bvh_width == 2u ? 2u : 4u;
+    kp.root = width == 4u ? s->ds.tlas4_root : s->ds.tlas2_root;
+    plan_jobs(iterations, params->current_iteration, tuning.samples_per_job,
+              uint64_t(kp.tiles_per_xcd) * kTile * kTile, &kp.chunk, &kp.chunks);
+    DevScene ds = s->ds;
+    ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
+    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
+    kp.epoch = epoch;
+    kp.split_pct = tuning.split_percent;
+    kp.pixels = params->width * params->height;
+    LaunchArgs a{kp, nullptr, nullptr, acc, rng, fb, stage, queues, kp.tiles};
+    reset_queues_kernel<<<1, 64, 0, st>>>(queues);
+    return launch_render<false>(s->features, width, tuning.workgroups_per_cu,
+                                static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a);
+}
+
+}  // namespace
+
+extern "C" {
+
 orn_status orn_render_async(orn_target *t, const orn_constant_params *params, uint32_t iterations)
 {
     if (!t) return ORN_ERROR_INVALID_HANDLE;
@@ -1254,38 +1331,46 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     if (params->width != t->width || params->height != t->height) return ORN_ERROR_SIZE_MISMATCH;
     orn_device_scene *s = t->scene;
     ORN_TRY(hipSetDevice(s->ctx->device));
-    KernelParams kp{};
-    kp.p = *params;
-    kp.iterations = iterations;
-    kp.tiles_x = blocks_for(t->width, kTile);
-    kp.tiles = kp.tiles_x * blocks_for(t->height, kTile);
-    kp.tiles_per_xcd = blocks_for(kp.tiles, kXcds);
-    kp.regroup = t->tuning.regroup_lanes;
-    kp.leaf_regroup = t->tuning.leaf_regroup_lanes;
-    kp.node_steps = t->tuning.node_steps;
-    const uint32_t width = t->tuning.bvh_width == 2u ? 2u : 4u;
-    kp.root = width == 4u ? s->ds.tlas4_root : s->ds.tlas2_root;
-    plan_jobs(iterations, params->current_iteration, t->tuning.samples_per_job,
-              uint64_t(kp.tiles_per_xcd) * kTile * kTile, &kp.chunk, &kp.chunks);
-    DevScene ds = s->ds;
-    ds.stack_slots = width == 4u ? s->stack_slots4 : s->stack_slots;
-    const size_t lds = size_t(ds.stack_slots) * kBlock * sizeof(uint32_t);
     hipStream_t st = s->ctx->stream;
     const uint32_t n = t->width * t->height;
     if (++t->epoch > 0xffffu) {  // tags wrap: clear the stage words
         fill_u32_kernel<<<blocks_for(n, 256), 256, 0, st>>>(t->stage, n, 0u);
         t->epoch = 1;
     }
-    kp.epoch = t->epoch;
-    kp.split_pct = t->tuning.split_percent;
-    kp.pixels = n;
-    LaunchArgs a{kp, nullptr, nullptr, t->acc, t->rng, t->fb, t->stage, t->queues, kp.tiles};
     ORN_TRY(hipEventRecord(t->ev_start, st));
-    reset_queues_kernel<<<1, 64, 0, st>>>(t->queues);
-    ORN_TRY(launch_render<false>(s->features, width, t->tuning.workgroups_per_cu,
-                                 static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a));
+    ORN_TRY(enqueue_render(s, params, iterations, t->tuning, t->acc, t->rng, t->fb, t->stage, t->queues, t->epoch,
+                           st));
     ORN_TRY(hipEventRecord(t->ev_stop, st));
     t->timed = true;
+    return ORN_OK;
+}
+
+orn_status orn_render_noalloc(orn_device_scene *s, const orn_constant_params *params, uint32_t iterations,
+                              const orn_tuning *tuning, float *accumulation, uint32_t *rng, float *framebuffer,
+                              void *stream)
+{
+    if (!s) return ORN_ERROR_INVALID_HANDLE;
+    if (!params || !accumulation || !rng || !framebuffer) return ORN_ERROR_INVALID_VALUE;
+    if (params->width == 0 || params->height == 0) return ORN_ERROR_INVALID_VALUE;
+    if (uint64_t(params->width) * params->height > ORN_RENDER_SCRATCH_PIXELS) return ORN_ERROR_SCRATCH_TOO_SMALL;
+    if (tuning && !tuning_valid(tuning)) return ORN_ERROR_INVALID_VALUE;
+    const orn_tuning tn = tuning ? normalized(*tuning) : default_tuning();
+    const int device = s->ctx->device;
+    if (device < 0 || device >= kMaxDevices) return ORN_ERROR_INVALID_DEVICE;
+    ORN_TRY(hipSetDevice(device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    ScratchSlot &sl = g_scratch[device];
+    if (!sl.ptr) return ORN_ERROR_INVALID_DEVICE;  // set when the device was opened
+    uint32_t *queues = sl.ptr;
+    uint32_t *stage = sl.ptr + kScratchQueueWords;
+    if (++sl.epoch > 0xffffu) {  // tags wrap: clear every stage word an earlier call may have left
+        fill_u32_kernel<<<blocks_for(ORN_RENDER_SCRATCH_PIXELS, 256), 256, 0, st>>>(stage, ORN_RENDER_SCRATCH_PIXELS,
+                                                                                   0u);
+        sl.epoch = 1;
+    }
+    ORN_TRY(enqueue_render(s, params, iterations, tn, reinterpret_cast<float4 *>(accumulation), rng,
+                           reinterpret_cast<float4 *>(framebuffer), stage, queues, sl.epoch, st));
     return ORN_OK;
 }
 
