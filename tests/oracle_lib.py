@@ -11,7 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "build", "libornament_oracle.so")
+# ORC_LIB selects another build of the oracle (the sanitizer build of
+# oracle/Makefile's sanitize target)
+LIB_PATH = os.environ.get("ORC_LIB") or os.path.join(ORACLE_DIR, "build", "libornament_oracle.so")
 
 f32, u32, P = C.c_float, C.c_uint32, C.POINTER
 vp = C.c_void_p

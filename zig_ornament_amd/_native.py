@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libornament_hip.so")
+# ORN_LIB selects another build of the library (the sanitizer build of
+# zig_ornament_amd/build.py --sanitize)
+LIB_PATH = os.environ.get("ORN_LIB") or os.path.join(HERE, "libornament_hip.so")
 
 u32 = C.c_uint32
 f32 = C.c_float

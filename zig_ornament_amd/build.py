@@ -30,22 +30,41 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-def build(verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
+# The sanitizer build (build(sanitize=True), `python build.py --sanitize`):
+# the same library with AddressSanitizer and UBSan on its host code only
+# (every sanitizer flag applies to the host pass; the GPU code is built as
+# usual), for the CPU test suite. It links clang's shared ASan runtime, which
+# the process must preload; select the build with ORN_LIB (see _native.py).
+SANITIZE_LIB = os.path.join(BUILD, "asan", "libornament_hip.so")
+HOST_SANITIZERS = [
+    "-Xarch_host", "-fsanitize=address",
+    "-Xarch_host", "-fsanitize=undefined",
+    "-Xarch_host", "-fno-sanitize-recover=undefined",
+    "-Xarch_host", "-fno-omit-frame-pointer",
+]
+SANITIZE_LINK = HOST_SANITIZERS[:4] + ["-Xarch_host", "-shared-libasan"]
+
+
+def build(verbose=True, sanitize=False):
+    out_dir = os.path.join(BUILD, "asan") if sanitize else BUILD
+    lib = SANITIZE_LIB if sanitize else LIB
+    os.makedirs(out_dir, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
+    common = COMMON + (["-g", *HOST_SANITIZERS] if sanitize else [])
     objs = []
     for src in ("orn_runtime.hip", "orn_nrc.hip"):
-        o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *COMMON, *DEVICE, *inc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", o])
+        o = os.path.join(out_dir, src.replace(".hip", ".o"))
+        _run([HIPCC, *common, *DEVICE, *inc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    host_o = os.path.join(BUILD, "orn_host.o")
-    _run([HIPCC, *COMMON, *inc, "-c", os.path.join(CSRC, "orn_host.cpp"), "-o", host_o])
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", *objs, host_o, "-o", tmp])
-    os.replace(tmp, LIB)
-    return LIB
+    host_o = os.path.join(out_dir, "orn_host.o")
+    _run([HIPCC, *common, *inc, "-c", os.path.join(CSRC, "orn_host.cpp"), "-o", host_o])
+    tmp = lib + ".tmp"
+    link = SANITIZE_LINK if sanitize else []
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", *link, *objs, host_o, "-o", tmp])
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    build()
+    build(sanitize="--sanitize" in sys.argv[1:])
     sys.exit(0)
