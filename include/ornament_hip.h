@@ -249,10 +249,7 @@ orn_status orn_target_wait(orn_target *target);
  *   before its wave looks for leaf and shading rounds again; fewer checks
  *   cost fewer instructions, while lanes that reach a leaf or finish their
  *   ray early wait longer (0 takes the default, 4).
- * lane_jobs (0, 1 or 2): pixel jobs a lane holds at once (0 takes the default,
- *   1). With 2 a lane traverses one job's ray while the other job's ray waits
- *   for shading or for the lane, so the lane idles only when neither has a
- *   ray to traverse (render targets only; a batch takes 0 or 1). */
+ * reserved: must be 0. */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
@@ -261,7 +258,7 @@ typedef struct orn_tuning {
     uint32_t samples_per_job;
     uint32_t split_percent;
     uint32_t node_steps;
-    uint32_t lane_jobs;
+    uint32_t reserved;
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);

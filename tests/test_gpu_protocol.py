@@ -136,8 +136,7 @@ def test_device_errors(device):
 
 def test_tuning_validation_and_defaults(device):
     """orn_target_set_tuning rejects out-of-range fields (node_steps above
-    16, more than 2 pixel jobs per lane) and stores node_steps 0 and
-    lane_jobs 0 as their defaults."""
+    16, a nonzero reserved word) and stores node_steps 0 as its default."""
     import ctypes as C
     from zig_ornament_amd import _native as N
     spec = W.cornell(width=16, height=16, spp=1)
@@ -155,20 +154,9 @@ def test_tuning_validation_and_defaults(device):
         assert e.value.status == 1, bad  # ORN_ERROR_INVALID_VALUE
     raw = N.Tuning()
     assert N.lib().orn_target_get_tuning(t._h, C.byref(raw)) == 0
-    assert raw.lane_jobs == 1
-    raw.lane_jobs = 3
+    raw.reserved = 1
     assert N.lib().orn_target_set_tuning(t._h, C.byref(raw)) == 1  # ORN_ERROR_INVALID_VALUE
     assert t.tuning()["node_steps"] == 16
-    raw.lane_jobs = 0
-    assert N.lib().orn_target_set_tuning(t._h, C.byref(raw)) == 0
-    assert t.tuning()["lane_jobs"] == 1
-    t.set_tuning(lane_jobs=2)
-    assert t.tuning()["lane_jobs"] == 2
-    b = zo.Batch(device, [(zo.Bvh(W.build_scene(spec)), 16, 16, 0)])
-    with pytest.raises(zo.OrnamentError) as e:  # batches render one job per lane
-        b.set_tuning(lane_jobs=2)
-    assert e.value.status == 1
-    b.destroy()
 
 
 def test_fractional_iteration_counter_matches_oracle(device):

@@ -500,323 +500,6 @@ __global__ void reset_queues_kernel(uint32_t *q)
     if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
 }
 
-// The per-pixel state of one job (the variables render_kernel keeps per lane).
-struct PixelJob {
-    uint32_t x, y, pix;
-    uint32_t job, chunks;  // index of the job within its pixel; jobs of the pixel (1: whole pixel)
-    uint32_t rng;
-    float4 acc;
-    float cur;
-    uint32_t left;  // samples left in the job
-    f3 thr;         // throughput of the current path
-    uint32_t bounce;
-};
-
-// Slot modes of render2_kernel. T (traversal slot): kSlotTrav, kSlotFinished,
-// kSlotBlocked, kSlotEmpty. S (shading slot): kSlotEmpty, kSlotDone, kSlotReady, kSlotFinished,
-// kSlotBlocked.
-enum SlotMode : uint32_t {
-    kSlotEmpty = 0u,     // no job (S: a new one is fetched at the next shading round)
-    kSlotDone = 1u,      // S: every band has run out
-    kSlotReady = 2u,     // S: the next ray is set up, waiting for the traversal slot
-    kSlotFinished = 3u,  // the ray's traversal ended; its hit waits for shading
-    kSlotBlocked = 4u,   // a job whose predecessor has not finished
-    kSlotTrav = 5u,      // T: traversing
-};
-
-// pathtracer.wgsl:64-79 + 95-161 + 81-93 like render_kernel, with two pixel
-// jobs per lane. A lane traverses one ray at a time (the T slot: ray,
-// traversal state and LDS stack column), while its second job (the S slot)
-// waits for shading or holds its next ray. When T's ray ends and S holds a
-// ready ray, the two slots trade places, so the lane goes on traversing
-// instead of idling until the wave's next shading round; the finished ray
-// then waits in S. Shading rounds (hit shading, accumulation, next camera
-// path, next job) run on S only, once `regroup` lanes have an S that needs
-// one, or once fewer than half as many lanes traverse as idle. A lane
-// idles only when neither of its jobs has a ray to traverse. Each pixel's
-// own sequence of operations is the reference's, so the images are those of
-// render_kernel, bit for bit. Jobs, bands, queues and stage words as in
-// render_kernel (single image).
-template <uint32_t F, uint32_t W>
-__global__ __launch_bounds__(kBlock) void render2_kernel(DevScene sc, KernelParams kp, float4 *__restrict__ acc_buf,
-                                                         uint32_t *__restrict__ rng_buf, float4 *__restrict__ fb,
-                                                         uint32_t *__restrict__ stage, uint32_t *__restrict__ queues)
-{
-    extern __shared__ uint32_t lds_stack[];
-    uint32_t *stack = lds_stack + threadIdx.x;
-    const uint32_t lane = threadIdx.x % 64u;
-    const __amdgpu_buffer_rsrc_t acc_rsrc =
-        __builtin_amdgcn_make_buffer_rsrc(acc_buf, 0, static_cast<int>(kp.pixels * 16u), kBufferWord3);
-    const orn_constant_params &p = kp.p;
-    const float t_min = p.ray_cast_epsilon;
-
-    PixelJob s{};  // S slot
-    Ray s_ray;
-    s_ray.o = mk(0.0f, 0.0f, 0.0f);
-    s_ray.d = s_ray.o;
-    bool s_hit = false;
-    Hit s_h{};
-    uint32_t s_mode = kSlotEmpty;
-    PixelJob t{};  // T slot
-    Ray t_ray = s_ray;
-    Traversal tr;
-    tr.addr = kLeafRef;  // a lane that is not traversing holds a leaf-flagged reference
-    tr.hit = false;
-    uint32_t t_mode = kSlotEmpty;
-    uint32_t release_pix = 0, release_tag = 0;  // stage word still to be written
-
-    // camera ray of S's new sample (camera.wgsl:12-23)
-    auto start_sample = [&]() {
-        const orn_camera &cam = p.camera;
-        const float u = (static_cast<float>(s.x) + rng_f32(s.rng)) / static_cast<float>(p.width - 1u);
-        const float v = (static_cast<float>(s.y) + rng_f32(s.rng)) / static_cast<float>(p.height - 1u);
-        const f3 disk = random_in_unit_disk(s.rng);
-        const f3 rd = cam.lens_radius * disk;
-        const f3 offset = rd.x * mk(cam.u[0], cam.u[1], cam.u[2]) + rd.y * mk(cam.v[0], cam.v[1], cam.v[2]);
-        const f3 origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
-        s_ray.o = origin + offset;
-        s_ray.d = ((mk(cam.lower_left_corner[0], cam.lower_left_corner[1], cam.lower_left_corner[2]) +
-                    u * mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2])) +
-                   v * mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]) - origin) -
-                  offset;
-        s.thr = mk(1.0f, 1.0f, 1.0f);
-        s.bounce = 0;
-    };
-    auto stage_tag = [&](uint32_t jobs_done) { return (kp.epoch << 16) | jobs_done; };
-    auto job_ready = [&]() { return s.job == 0u || load_coherent(stage + s.pix) == stage_tag(s.job); };
-    // end of S's job: as render_kernel's end_job
-    auto end_job = [&]() {
-        if (s.job + 1u < s.chunks) {  // handed on; released at the wave's next flush
-            store_coherent(acc_rsrc, s.pix, s.acc);
-            store_coherent(rng_buf + s.pix, s.rng);
-            if (release_tag != 0u) {  // a release is still pending: write it now
-                release_wait();
-                store_coherent(stage + release_pix, release_tag);
-            }
-            release_pix = s.pix;
-            release_tag = stage_tag(s.job + 1u);
-            return;
-        }
-        acc_buf[s.pix] = s.acc;  // the pixel's last job: read next by a later launch
-        rng_buf[s.pix] = s.rng;
-        float4 o;
-        o.x = fminf(fmaxf(s.acc.x / s.cur, 0.0f), 1.0f);
-        o.y = fminf(fmaxf(s.acc.y / s.cur, 0.0f), 1.0f);
-        o.z = fminf(fmaxf(s.acc.z / s.cur, 0.0f), 1.0f);
-        o.w = fminf(fmaxf(s.acc.w / s.cur, 0.0f), 1.0f);
-        o.x = pow_f(o.x, p.inverted_gamma);
-        o.y = pow_f(o.y, p.inverted_gamma);
-        o.z = pow_f(o.z, p.inverted_gamma);
-        const uint32_t row = p.flip_y < 1u ? s.y : p.height - s.y - 1u;
-        fb[row * p.width + s.x] = o;
-    };
-    // accumulate S's finished sample (if any), finish samples that need no
-    // traversal (depth exhausted), and leave S ready with its next ray, or
-    // empty when its job has ended
-    auto next_ray = [&](bool done, f3 color) {
-        for (;;) {
-            if (done) {
-                s.cur = s.cur + 1.0f;  // State.nextIteration
-                if (s.cur > 1.0f) {
-                    s.acc.x = s.acc.x + color.x;
-                    s.acc.y = s.acc.y + color.y;
-                    s.acc.z = s.acc.z + color.z;
-                    s.acc.w = s.acc.w + 1.0f;
-                } else {
-                    s.acc = make_float4(color.x, color.y, color.z, 1.0f);
-                }
-                if (--s.left == 0u) {
-                    end_job();
-                    s_mode = kSlotEmpty;
-                    return;
-                }
-                start_sample();
-            }
-            if (s.bounce >= p.depth) {  // depth exhausted: the throughput is returned
-                color = s.thr;
-                done = true;
-                continue;
-            }
-            s_mode = kSlotReady;
-            return;
-        }
-    };
-    auto begin_job = [&]() {
-        if (s.job == 0u) {  // state left by an earlier launch
-            s.rng = rng_buf[s.pix];
-            s.acc = acc_buf[s.pix];
-        } else {  // state handed on within this launch
-            acquire_order();
-            s.rng = load_coherent(rng_buf + s.pix);
-            s.acc = load_coherent(acc_rsrc, s.pix);
-        }
-        const uint32_t spp = kp.iterations > 1u ? kp.iterations : 1u;
-        const uint32_t chunk = s.chunks == 1u ? spp : kp.chunk;
-        const uint32_t first = s.job * chunk;
-        s.cur = first == 0u ? p.current_iteration : p.current_iteration + static_cast<float>(first);
-        s.left = min(chunk, spp - first);
-        start_sample();
-        next_ray(false, s.thr);
-    };
-    auto flush_releases = [&]() {
-        if (lanes_where(release_tag != 0u) != 0u) {
-            release_wait();
-            if (release_tag != 0u) {
-                store_coherent(stage + release_pix, release_tag);
-                release_tag = 0u;
-            }
-        }
-    };
-    // T and S trade places (see the kernel comment); a ready ray starts its
-    // traversal, a finished ray takes its hit record along to S
-    auto swap_slots = [&]() {
-        const uint32_t sm = s_mode, tm = t_mode;
-        const PixelJob pj = t;
-        t = s;
-        s = pj;
-        const Ray r = t_ray;
-        t_ray = s_ray;
-        s_ray = r;
-        if (tm == kSlotFinished) {
-            s_hit = tr.hit;
-            s_h = tr.h;
-        }
-        s_mode = tm;
-        if (sm == kSlotReady) {
-            trav_begin(kp.root, t_ray, tr);
-            t_mode = kSlotTrav;
-        } else {
-            t_mode = sm == kSlotBlocked ? kSlotBlocked : kSlotEmpty;  // from an S that held nothing to traverse
-        }
-    };
-    // the trades every lane whose T is not traversing makes (a finished ray
-    // must reach S to be shaded; a blocked job waits in S for its check)
-    auto trade = [&]() {
-        const bool sw = t_mode != kSlotTrav && (s_mode == kSlotReady || (t_mode == kSlotFinished && s_mode != kSlotFinished) ||
-                                            (t_mode == kSlotBlocked && s_mode == kSlotDone));
-        if (lanes_where(sw) != 0u && sw) swap_slots();
-    };
-
-    const uint32_t max_chunks = kp.chunks;
-    const uint32_t split_pct = kp.split_pct;
-    auto band_pixels = [&](uint32_t b) -> uint32_t {
-        const uint32_t first = b * kp.tiles_per_xcd;
-        return first >= kp.tiles ? 0u : min(kp.tiles_per_xcd, kp.tiles - first) * (kTile * kTile);
-    };
-    uint32_t band = blockIdx.x % kXcds;  // wave-uniform
-    uint32_t bands_left = kXcds - 1u;
-    uint32_t band_px = band_pixels(band);
-    uint32_t band_split = band_px / 100u * split_pct + band_px % 100u * split_pct / 100u;
-    uint32_t band_jobs = band_px - band_split + band_split * max_chunks;
-    const uint32_t regroup = kp.regroup;
-    const uint32_t leaf_regroup = kp.leaf_regroup;
-    const uint32_t node_steps = kp.node_steps;
-    for (;;) {
-        // up to `node_steps` internal nodes per traversing lane, then the
-        // leaves in rounds of at least `leaf_regroup` lanes (render_kernel)
-        for (uint32_t k = 0; k < node_steps; k++) {
-            if (!(tr.addr & kLeafRef) && !trav_internal<F, W>(sc, t_min, t_ray, stack, kBlock, tr)) {
-                t_mode = kSlotFinished;
-                tr.addr = kLeafRef;
-            }
-        }
-        const uint64_t tracing = lanes_where(t_mode == kSlotTrav);
-        const uint64_t at_inner = lanes_where(!(tr.addr & kLeafRef));
-        const uint64_t at_leaf = tracing & ~at_inner;
-        if (at_leaf != 0u && (static_cast<uint32_t>(__popcll(at_leaf)) >= leaf_regroup || at_inner == 0u)) {
-            if (t_mode == kSlotTrav && (tr.addr & kLeafRef) && !trav_leaf<F, W>(sc, t_min, t_ray, stack, kBlock, tr)) {
-                t_mode = kSlotFinished;
-                tr.addr = kLeafRef;
-            }
-        }
-        trade();
-        const uint64_t traversing = lanes_where(t_mode == kSlotTrav);
-        const uint64_t waiting = lanes_where(s_mode != kSlotReady && s_mode != kSlotDone);
-        if (waiting == 0u && traversing == 0u) break;  // every lane: T empty, S done
-        const uint32_t n_trav = static_cast<uint32_t>(__popcll(traversing));
-        if (static_cast<uint32_t>(__popcll(waiting)) < regroup && traversing != 0u && 2u * n_trav >= 64u - n_trav)
-            continue;
-        // shading round on S
-        flush_releases();
-        if (s_mode == kSlotBlocked) {
-            if (job_ready()) begin_job();
-        } else if (s_mode == kSlotFinished) {
-            s.bounce++;
-            bool done;
-            f3 color;
-            if (!s_hit) {
-                const f3 ud = normalize(s_ray.d);
-                const float tt = 0.5f * (ud.y + 1.0f);
-                const float a = (1.0f - tt) * 1.0f;
-                done = true;
-                color = s.thr * mk(a + tt * 0.5f, a + tt * 0.7f, a + tt * 1.0f);
-            } else {
-                Ray next;
-                f3 w;
-                done = !shade<F>(sc, p.textures_count, 0u, s_ray, s_h, s.rng, next, w);
-                if (done) {
-                    color = s.thr * w;
-                } else {
-                    s_ray = next;
-                    s.thr = s.thr * w;
-                    color = s.thr;
-                }
-            }
-            next_ray(done, color);
-        }
-        // S slots without a job take the next ones of the wave's band, one
-        // atomic per wave (render_kernel's fetch)
-        for (;;) {
-            const uint64_t needy = lanes_where(s_mode == kSlotEmpty);
-            if (needy == 0u) break;
-            const int leader = __ffsll(static_cast<unsigned long long>(needy)) - 1;
-            const uint32_t count = static_cast<uint32_t>(__popcll(needy));
-            uint32_t base = 0;
-            if (lane == static_cast<uint32_t>(leader)) base = atomicAdd(queues + band, count);
-            base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), leader));
-            if (s_mode == kSlotEmpty) {
-                const uint32_t q = base + static_cast<uint32_t>(__popcll(needy & ((1ull << lane) - 1ull)));
-                if (q >= band_jobs) {
-                    if (bands_left == 0u) s_mode = kSlotDone;
-                } else {
-                    const uint32_t bulk = band_px - band_split;
-                    uint32_t c = 0, qp = q;
-                    if (q >= bulk) {
-                        c = (q - bulk) / band_split;
-                        qp = bulk + (q - bulk - c * band_split);
-                    }
-                    const uint32_t within = qp % (kTile * kTile);
-                    const uint32_t tile = band * kp.tiles_per_xcd + qp / (kTile * kTile);
-                    s.x = (tile % kp.tiles_x) * kTile + within % kTile;
-                    s.y = (tile / kp.tiles_x) * kTile + within / kTile;
-                    const bool split = qp >= bulk;
-                    if (s.x < p.width && s.y < p.height && (!split || c < kp.chunks)) {
-                        s.pix = s.y * p.width + s.x;
-                        s.chunks = split ? kp.chunks : 1u;
-                        s.job = c;
-                        if (job_ready()) {
-                            begin_job();
-                        } else {
-                            s_mode = kSlotBlocked;
-                        }
-                    }
-                }
-            }
-            if (base + count > band_jobs && bands_left != 0u) {  // the band ran out (wave-uniform)
-                band = band + 1u == kXcds ? 0u : band + 1u;
-                bands_left--;
-                band_px = band_pixels(band);
-                band_split = band_px / 100u * split_pct + band_px % 100u * split_pct / 100u;
-                band_jobs = band_px - band_split + band_split * max_chunks;
-            }
-        }
-        // two blocked jobs: the one in T gets S's place for the next check
-        if (t_mode == kSlotBlocked && s_mode == kSlotBlocked) swap_slots();
-        trade();
-    }
-    flush_releases();  // no successor may wait on a release this wave holds
-}
-
 template <uint32_t W>
 __global__ __launch_bounds__(kBlock) void trace_kernel(DevScene sc, float t_min,
                                                        const float *__restrict__ rays, uint32_t n,
@@ -1142,7 +825,6 @@ orn_tuning default_tuning()
     t.samples_per_job = 16;
     t.split_percent = 0;
     t.node_steps = 4;
-    t.lane_jobs = 1;
     return t;
 }
 
@@ -1151,14 +833,13 @@ bool tuning_valid(const orn_tuning *t)
     return t && t->regroup_lanes >= 1 && t->regroup_lanes <= 64 && t->workgroups_per_cu <= 64 &&
            (t->bvh_width == 2u || t->bvh_width == 4u) && t->leaf_regroup_lanes >= 1 &&
            t->leaf_regroup_lanes <= 64 && t->samples_per_job <= 0xffffu && t->split_percent <= 100u &&
-           t->node_steps <= 16u && t->lane_jobs <= 2u;
+           t->node_steps <= 16u && t->reserved == 0u;
 }
 
 // node_steps 0 stands for the default
 orn_tuning normalized(orn_tuning t)
 {
     if (t.node_steps == 0u) t.node_steps = default_tuning().node_steps;
-    if (t.lane_jobs == 0u) t.lane_jobs = 1u;
     return t;
 }
 
@@ -1220,48 +901,6 @@ hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_
     render_kernel<F, W, BATCH>
         <<<grid, kBlock, lds, st>>>(ds, a.kp, a.items, a.work, a.acc, a.rng, a.fb, a.stage, a.queues);
     return hipGetLastError();
-}
-
-// render2_kernel (two pixel jobs per lane, single image), same grid rule
-template <uint32_t F, uint32_t W>
-hipError_t launch_render2_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
-                            const LaunchArgs &a)
-{
-    if (per_cu == 0) {
-        int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render2_kernel<F, W>, kBlock, lds) != hipSuccess ||
-            blocks < 1) {
-            (void)hipGetLastError();  // the launch below reports its own errors
-            blocks = 1;
-        }
-        per_cu = static_cast<uint32_t>(blocks);
-    }
-    uint32_t grid = cus * per_cu;
-    grid = std::min(grid, blocks_for(a.total_tiles, kXcds) * kXcds);
-    grid = std::max(kXcds, grid / kXcds * kXcds);
-    render2_kernel<F, W><<<grid, kBlock, lds, st>>>(ds, a.kp, a.acc, a.rng, a.fb, a.stage, a.queues);
-    return hipGetLastError();
-}
-
-hipError_t launch_render2(uint32_t features, uint32_t width, uint32_t per_cu, uint32_t cus, size_t lds,
-                          hipStream_t st, const DevScene &ds, const LaunchArgs &a)
-{
-#define ORN_CASE(F)                                                                  \
-    case F:                                                                          \
-        return width == 4u ? launch_render2_f<F, 4>(per_cu, cus, lds, st, ds, a)    \
-                           : launch_render2_f<F, 2>(per_cu, cus, lds, st, ds, a);
-    switch (features & 7u) {
-        ORN_CASE(0u)
-        ORN_CASE(1u)
-        ORN_CASE(2u)
-        ORN_CASE(3u)
-        ORN_CASE(4u)
-        ORN_CASE(5u)
-        ORN_CASE(6u)
-        ORN_CASE(7u)
-    }
-#undef ORN_CASE
-    return hipErrorInvalidValue;
 }
 
 template <bool BATCH>
@@ -1692,9 +1331,6 @@ hipError_t enqueue_render(orn_device_scene *s, const orn_constant_params *params
     kp.pixels = params->width * params->height;
     LaunchArgs a{kp, nullptr, nullptr, acc, rng, fb, stage, queues, kp.tiles};
     reset_queues_kernel<<<1, 64, 0, st>>>(queues);
-    if (tuning.lane_jobs == 2u)
-        return launch_render2(s->features, width, tuning.workgroups_per_cu,
-                              static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a);
     return launch_render<false>(s->features, width, tuning.workgroups_per_cu,
                                 static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a);
 }
@@ -2333,7 +1969,7 @@ orn_status orn_batch_render(orn_batch *b, const orn_constant_params *params, con
 orn_status orn_batch_set_tuning(orn_batch *b, const orn_tuning *tuning)
 {
     if (!b) return ORN_ERROR_INVALID_HANDLE;
-    if (!tuning_valid(tuning) || tuning->lane_jobs > 1u) return ORN_ERROR_INVALID_VALUE;
+    if (!tuning_valid(tuning)) return ORN_ERROR_INVALID_VALUE;
     const orn_tuning n = normalized(*tuning);
     if (std::memcmp(&n, &b->tuning, sizeof(orn_tuning)) != 0) {
         ORN_TRY(hipSetDevice(b->scene->ctx->device));
@@ -2539,7 +2175,6 @@ bool cached_tuning(const char *path, const std::string &key, orn_tuning *t, floa
             else if (f.first == "samples_per_job") r.samples_per_job = v;
             else if (f.first == "split_percent") r.split_percent = v;
             else if (f.first == "node_steps") r.node_steps = v;
-            else if (f.first == "lane_jobs") r.lane_jobs = v;
             else if (f.first == "ms") *ms = static_cast<float>(f.second);
         }
         if (!(ok[0] && ok[1] && ok[2] && ok[3])) return false;
@@ -2562,7 +2197,6 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
         {"samples_per_job", t.samples_per_job},
         {"split_percent", t.split_percent},
         {"node_steps", t.node_steps},
-        {"lane_jobs", t.lane_jobs},
         {"ms", ms}};
     bool found = false;
     for (auto &e : c)
