@@ -392,7 +392,8 @@ typedef struct orn_nrc_config {
     uint32_t max_resolution;   /* finest grid (default 2048) */
     uint32_t terminate_bounce; /* bounce whose hit is queried (default 1; 0: primary hit) */
     uint32_t train_paths;      /* training paths per orn_nrc_train (default 65536) */
-    uint32_t train_batch;      /* samples per Adam step, a multiple of 64 (default 16384) */
+    uint32_t train_batch;      /* samples per Adam step, a multiple of 64 (default 16384); a step
+                                * of a frame with fewer training records takes each record once */
     float learning_rate;       /* default 1e-2 */
     float beta1, beta2, epsilon; /* default 0.9, 0.99, 1e-15 */
     uint32_t seed;             /* initial parameters (default 1) */

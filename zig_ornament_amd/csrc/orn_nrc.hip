@@ -182,6 +182,20 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// gfx950's v_mfma_f32_16x16x32_bf16: twice the K of the instruction above per
+// issue. Lane l supplies A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16]
+// (j = 0..7) and holds D as above.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 row8(const uint16_t *p)  // 8 consecutive (16-byte aligned)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
+}
+
 __device__ __forceinline__ bf16x4 row4(const uint16_t *p)  // 4 consecutive (8-byte aligned)
 {
     const uint2 v = *reinterpret_cast<const uint2 *>(p);
@@ -213,10 +227,10 @@ __device__ __forceinline__ void layer_forward(const uint16_t *Wt, uint32_t n_til
     const uint32_t r = lane % 16u, q = lane / 16u;
     for (uint32_t nt = 0; nt < n_tiles; nt++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (uint32_t ks = 0; ks < 4u; ks++) {
-            const bf16x4 a = row4(Wt + (nt * 16u + r) * kStride + ks * 16u + 4u * q);
-            const bf16x4 b = row4(X + (it * 16u + r) * kStride + ks * 16u + 4u * q);
-            acc = mfma(a, b, acc);
+        for (uint32_t ks = 0; ks < 2u; ks++) {  // K = 64 in two 32-wide steps
+            const bf16x8 a = row8(Wt + (nt * 16u + r) * kStride + ks * 32u + 8u * q);
+            const bf16x8 b = row8(X + (it * 16u + r) * kStride + ks * 32u + 8u * q);
+            acc = mfma32(a, b, acc);
         }
         // acc[j] = Y[it*16 + r][nt*16 + 4q + j]
         if (out) {
@@ -363,9 +377,10 @@ __global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __ha
 
 // ---------------------------------------------------------------------------
 // Training: forward + backward of one 64-sample tile per workgroup over a
-// batch of `batch` samples, records first, first + 1, ... taken modulo the
-// record count (a frame with fewer records than a step needs reuses them;
-// none: an empty step). Writes the tile's weight gradient to
+// batch of min(batch, record count) samples, records first, first + 1, ...
+// taken modulo the record count (a frame with fewer records than a step
+// needs steps over each of them once; none: an empty step; workgroups past
+// the batch write nothing). Writes the tile's weight gradient to
 // partials[blockIdx.x], the tile's loss to loss_sum, and for batch sample b
 // its unit position (sample_u[b]), the loss gradient of its grid features
 // (sample_dx[l * dx_stride + b], both features of level l) and the ranges
@@ -389,9 +404,10 @@ __global__ __launch_bounds__(kBlock) void nrc_train_kernel(NrcGrid g, const __ha
     __shared__ __attribute__((aligned(16))) uint16_t s_dy[2][kTileRows * kStride];
     __shared__ float s_dx0[kTileRows * kGridFeatures];
     const uint32_t total = min(count_ptr ? *count_ptr : count_max, count_max);
-    const uint32_t n_batch = total ? batch : 0u;
+    const uint32_t n_batch = min(batch, total);  // a frame with fewer records steps over all of them
     const uint32_t tile_rel = blockIdx.x * kTileRows;
-    const uint32_t valid = n_batch > tile_rel ? min(n_batch - tile_rel, kTileRows) : 0u;
+    if (tile_rel >= n_batch) return;  // no sample: nrc_mlp_kernel does not read this workgroup's partial
+    const uint32_t valid = min(n_batch - tile_rel, kTileRows);
     const uint32_t tile_base = first + tile_rel;
     const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
     const uint32_t r = lane % 16u, q = lane / 16u;
@@ -515,15 +531,19 @@ enum GradMode : int {
 constexpr uint32_t kMlpSlices = 16;
 static_assert(kMlpParams % 64u == 0u, "MLP parameters per workgroup");
 __global__ __launch_bounds__(64u * kMlpSlices) void nrc_mlp_kernel(const float *__restrict__ partials, uint32_t groups,
-                                                                   AdamStep a, float *__restrict__ w,
-                                                                   float *__restrict__ grad, float *__restrict__ m,
-                                                                   float *__restrict__ v, uint16_t *__restrict__ wt,
-                                                                   int mode)
+                                                                   const uint32_t *__restrict__ count_ptr,
+                                                                   uint32_t count_max, uint32_t batch, AdamStep a,
+                                                                   float *__restrict__ w, float *__restrict__ grad,
+                                                                   float *__restrict__ m, float *__restrict__ v,
+                                                                   uint16_t *__restrict__ wt, int mode)
 {
     __shared__ float s_sum[kMlpSlices][64];
     const uint32_t lane = threadIdx.x % 64u, slice = threadIdx.x / 64u;
     const uint32_t j = blockIdx.x * 64u + lane;
     if (mode != kRefresh) {
+        // the training workgroups that held samples (nrc_train_kernel's batch)
+        const uint32_t total = min(count_ptr ? *count_ptr : count_max, count_max);
+        groups = min(groups, (min(batch, total) + kTileRows - 1u) / kTileRows);
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         uint32_t gi = slice;
         for (; gi + 3u * kMlpSlices < groups; gi += 4u * kMlpSlices)
@@ -634,7 +654,7 @@ __global__ __launch_bounds__(kGridBlock) void nrc_grid_kernel(NrcGrid g, GridPar
     for (uint32_t i = threadIdx.x; i < kPartEntries; i += kGridBlock) s_acc[i] = make_float2(0.0f, 0.0f);
     __syncthreads();
     const uint32_t total = min(count_ptr ? *count_ptr : count_max, count_max);
-    const uint32_t n_batch = total ? batch : 0u;
+    const uint32_t n_batch = min(batch, total);
     const uint32_t per = (n_batch + chunks - 1u) / chunks;
     const uint32_t b_end = min(n_batch, (chunk + 1u) * per);
     const float2 *dx = sample_dx + size_t(l) * dx_stride;
@@ -1058,31 +1078,28 @@ __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(const NrcRecord *__
 
 // render() bookkeeping of one sample per pixel (pathtracer.wgsl:64-79):
 // current_iteration + 1, then overwrite or add.
-__global__ void nrc_accumulate_kernel(const float4 *__restrict__ partial, float4 *__restrict__ acc, uint32_t n,
-                                      float cur)
+// One sample per pixel into the accumulation (pathtracer.wgsl:64-79); with
+// `post` (the render's last iteration) also the framebuffer
+// (pathtracer.wgsl:81-93) from the new accumulation.
+__global__ void nrc_accumulate_kernel(const float4 *__restrict__ partial, float4 *__restrict__ acc, uint32_t width,
+                                      uint32_t height, float cur, int post, float inv_gamma, uint32_t flip_y,
+                                      float4 *__restrict__ fb)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= width * height) return;
     const float4 c = partial[i];
+    float4 a;
     if (cur > 1.0f) {
-        float4 a = acc[i];
+        a = acc[i];
         a.x = a.x + c.x;
         a.y = a.y + c.y;
         a.z = a.z + c.z;
         a.w = a.w + 1.0f;
-        acc[i] = a;
     } else {
-        acc[i] = make_float4(c.x, c.y, c.z, 1.0f);
+        a = make_float4(c.x, c.y, c.z, 1.0f);
     }
-}
-
-// pathtracer.wgsl:81-93
-__global__ void nrc_post_kernel(const float4 *__restrict__ acc, float4 *__restrict__ fb, uint32_t width,
-                                uint32_t height, float cur, float inv_gamma, uint32_t flip_y)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= width * height) return;
-    const float4 a = acc[i];
+    acc[i] = a;
+    if (!post) return;
     float4 o;
     o.x = pow_f(fminf(fmaxf(a.x / cur, 0.0f), 1.0f), inv_gamma);
     o.y = pow_f(fminf(fmaxf(a.y / cur, 0.0f), 1.0f), inv_gamma);
@@ -1215,8 +1232,8 @@ AdamStep adam_step(const orn_nrc *n)
 hipError_t refresh_params(orn_nrc *n, hipStream_t st)
 {
     const AdamStep a = adam_step(n);
-    nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(nullptr, 0u, a, n->mlp, n->mlp_grad, n->mlp_m,
-                                                                 n->mlp_v, n->mlp_wt, kRefresh);
+    nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(nullptr, 0u, nullptr, 0u, 0u, a, n->mlp,
+                                                                 n->mlp_grad, n->mlp_m, n->mlp_v, n->mlp_wt, kRefresh);
     const uint64_t g = n->grid_params;
     nrc_grid_half_kernel<<<static_cast<uint32_t>((g + 255) / 256), 256, 0, st>>>(n->grid_w, n->grid_half, g);
     return hipGetLastError();
@@ -1246,8 +1263,9 @@ hipError_t apply_gradients(orn_nrc *n, const uint32_t *count_ptr, uint32_t count
     const uint32_t groups = blocks_for(batch, kTileRows);
     if (groups > n->partial_groups) return hipErrorInvalidValue;
     const AdamStep a = adam_step(n);
-    nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(n->partials, groups, a, n->mlp, n->mlp_grad,
-                                                                 n->mlp_m, n->mlp_v, n->mlp_wt, mode);
+    nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(n->partials, groups, count_ptr, count_max, batch,
+                                                                 a, n->mlp, n->mlp_grad, n->mlp_m, n->mlp_v,
+                                                                 n->mlp_wt, mode);
     GridUpdate up;
     up.a = a;
     up.w = n->grid_w;
@@ -1574,10 +1592,10 @@ orn_status enqueue_render(orn_nrc *n, orn_target *t, const orn_constant_params *
             n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, n->queries, n->counters, px, 1,
             nullptr, n->partial, nullptr);
         cur = cur + 1.0f;
-        nrc_accumulate_kernel<<<blocks_for(px, 256), 256, 0, st>>>(n->partial, t->acc, px, cur);
+        nrc_accumulate_kernel<<<blocks_for(px, 256), 256, 0, st>>>(n->partial, t->acc, t->width, t->height, cur,
+                                                                   it + 1u == iters ? 1 : 0, params->inverted_gamma,
+                                                                   params->flip_y, t->fb);
     }
-    nrc_post_kernel<<<blocks_for(px, 256), 256, 0, st>>>(t->acc, t->fb, t->width, t->height, cur,
-                                                         params->inverted_gamma, params->flip_y);
     ORN_TRY(hipGetLastError());
     return ORN_OK;
 }
@@ -1605,7 +1623,8 @@ orn_status enqueue_steps(orn_nrc *n, uint32_t steps, bool first_trained = false)
     const uint32_t b = n->cfg.train_batch;
     for (uint32_t k = 0; k < steps; k++) {
         // step k takes records k*b, k*b + 1, ... of this frame's samples,
-        // modulo their count (an empty frame makes a step with zero gradient)
+        // modulo their count, min(b, count) of them (an empty frame makes a
+        // step with zero gradient)
         n->step++;
         if (k > 0u || !first_trained)
             ORN_TRY(train_gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
