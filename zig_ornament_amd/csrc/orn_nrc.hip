@@ -327,17 +327,57 @@ __device__ __forceinline__ void load_weights(const uint16_t *__restrict__ wt, ui
     for (uint32_t i = threadIdx.x; i < kWtElems / 8u; i += kBlock) dst[i] = src[i];
 }
 
+// Where a render's samples go: the target's accumulation and, after the
+// render's last sample (post), its framebuffer. cur is State.current_iteration
+// after this sample.
+struct PixelSink {
+    float4 *acc;
+    float4 *fb;
+    uint32_t width, height;
+    float cur;
+    int post;
+    float inv_gamma;
+    uint32_t flip_y;
+};
+
+// One finished sample of pixel i: State.nextIteration's accumulation
+// (pathtracer.wgsl:64-79) and, on the last sample, main_post_processing
+// (pathtracer.wgsl:81-93).
+__device__ __forceinline__ void add_sample(const PixelSink &o, uint32_t i, f3 c)
+{
+    float4 a;
+    if (o.cur > 1.0f) {
+        a = o.acc[i];
+        a.x = a.x + c.x;
+        a.y = a.y + c.y;
+        a.z = a.z + c.z;
+        a.w = a.w + 1.0f;
+    } else {
+        a = make_float4(c.x, c.y, c.z, 1.0f);
+    }
+    o.acc[i] = a;
+    if (!o.post) return;
+    float4 v;
+    v.x = pow_f(fminf(fmaxf(a.x / o.cur, 0.0f), 1.0f), o.inv_gamma);
+    v.y = pow_f(fminf(fmaxf(a.y / o.cur, 0.0f), 1.0f), o.inv_gamma);
+    v.z = pow_f(fminf(fmaxf(a.z / o.cur, 0.0f), 1.0f), o.inv_gamma);
+    v.w = fminf(fmaxf(a.w / o.cur, 0.0f), 1.0f);
+    const uint32_t x = i % o.width, y = i / o.width;
+    o.fb[o.flip_y < 1u ? i : o.width * (o.height - y - 1u) + x] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Inference: predictions for n records, 64 per workgroup tile (persistent).
-// mode 0: raw predictions to out_rgb[3 i]; mode 1: partial[pixel] += weight *
-// max(prediction, 0) (one query per pixel per launch).
+// mode 0: raw predictions to out_rgb[3 i]; mode 1: the sample of the
+// record's pixel, weight * max(prediction, 0), into `out` (a path that
+// reaches its query has gathered no light before it: emitters end paths).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __half2 *__restrict__ table,
                                                            const uint16_t *__restrict__ wt,
                                                            const NrcRecord *__restrict__ recs,
                                                            const uint32_t *__restrict__ count_ptr, uint32_t count_max,
-                                                           int mode, float *__restrict__ out_rgb,
-                                                           float4 *__restrict__ partial, float *__restrict__ inputs)
+                                                           int mode, float *__restrict__ out_rgb, PixelSink out,
+                                                           float *__restrict__ inputs)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtElems];
     __shared__ __attribute__((aligned(16))) uint16_t s_x[2][kTileRows * kStride];
@@ -366,10 +406,9 @@ __global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __ha
                 out_rgb[3u * i + 2u] = y[2];
             } else {
                 const NrcRecord &rc = recs[i];
-                float4 &pv = partial[rc.pixel];
-                pv.x = pv.x + rc.weight[0] * fmaxf(y[0], 0.0f);
-                pv.y = pv.y + rc.weight[1] * fmaxf(y[1], 0.0f);
-                pv.z = pv.z + rc.weight[2] * fmaxf(y[2], 0.0f);
+                add_sample(out, rc.pixel,
+                           mk(rc.weight[0] * fmaxf(y[0], 0.0f), rc.weight[1] * fmaxf(y[1], 0.0f),
+                              rc.weight[2] * fmaxf(y[2], 0.0f)));
             }
         }
     }
@@ -520,6 +559,8 @@ enum GradMode : int {
     kRefresh = 0,   // no gradient: only rewrite the working copies from the masters
     kGradient = 1,  // store the gradient (orn_nrc_gradients)
     kUpdate = 2,    // one Adam step
+    kUpdateMasters = 3,  // one Adam step on the fp32 masters only; the working
+                         // copies are rewritten later (refresh_params)
 };
 
 // MLP parameters: dW = sum over the training workgroups' partials
@@ -558,7 +599,7 @@ __global__ __launch_bounds__(64u * kMlpSlices) void nrc_mlp_kernel(const float *
         float gj = 0.0f;
         for (uint32_t k = 0; k < kMlpSlices; k++) gj = gj + s_sum[k][lane];
         grad[j] = gj;
-        if (mode == kUpdate) {
+        if (mode == kUpdate || mode == kUpdateMasters) {
             float mj = m[j], vj = v[j];
             p = adam(p, gj, mj, vj, a);
             m[j] = mj;
@@ -566,6 +607,7 @@ __global__ __launch_bounds__(64u * kMlpSlices) void nrc_mlp_kernel(const float *
             w[j] = p;
         }
     }
+    if (mode == kUpdateMasters) return;
     uint32_t dst;
     if (j < kOutOffset) {
         const uint32_t l = j / (kIn * kW), k = (j / kW) % kIn, n = j % kW;
@@ -577,11 +619,16 @@ __global__ __launch_bounds__(64u * kMlpSlices) void nrc_mlp_kernel(const float *
     wt[dst] = to_bf16(p);
 }
 
-// fp16 working copy of the grid masters
-__global__ void nrc_grid_half_kernel(const float *__restrict__ w, __half *__restrict__ half_w, uint64_t n)
+// fp16 working copy of the grid masters, four parameters per thread (n is a
+// multiple of 4: whole tables of 2^log2_table two-feature entries)
+__global__ void nrc_grid_half_kernel(const float4 *__restrict__ w, uint2 *__restrict__ half_w, uint64_t n4)
 {
     const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < n) half_w[j] = __float2half_rn(w[j]);
+    if (j >= n4) return;
+    const float4 x = w[j];
+    const __half2 lo = __halves2half2(__float2half_rn(x.x), __float2half_rn(x.y));
+    const __half2 hi = __halves2half2(__float2half_rn(x.z), __float2half_rn(x.w));
+    half_w[j] = make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
 }
 
 // Grid gradient and update without global atomics. A workgroup owns
@@ -633,7 +680,7 @@ __device__ __forceinline__ void finish_entry(const GridUpdate &u, size_t j, floa
         u.m[j + f] = mj;
         u.v[j + f] = vj;
         u.w[j + f] = p;
-        u.half_w[j + f] = __float2half_rn(p);
+        if (u.mode == kUpdate) u.half_w[j + f] = __float2half_rn(p);
     }
 }
 
@@ -826,8 +873,7 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count)
 // (k = 0 has its own kernel, nrc_primary_kernel.)
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_constant_params p, uint32_t k,
-                                                           uint32_t *__restrict__ rng_buf,
-                                                           float4 *__restrict__ partial,
+                                                           uint32_t *__restrict__ rng_buf, PixelSink out,
                                                            NrcRecord *__restrict__ queries,
                                                            uint32_t *__restrict__ query_count)
 {
@@ -877,7 +923,7 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
             }
         }
         if (!done) color = thr;
-        partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+        if (!query) add_sample(out, pix, color);  // else the inference kernel adds it
         rng_buf[pix] = rng;
     }
     const uint32_t slot = wave_append(query, query_count);
@@ -895,8 +941,7 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
 // traverses alone (LDS stack).
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_constant_params p,
-                                                             uint32_t *__restrict__ rng_buf,
-                                                             float4 *__restrict__ partial,
+                                                             uint32_t *__restrict__ rng_buf, PixelSink out,
                                                              NrcRecord *__restrict__ queries,
                                                              uint32_t *__restrict__ query_count)
 {
@@ -949,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_co
                 }
             }
         }
-        partial[pix] = make_float4(color.x, color.y, color.z, 0.0f);
+        if (!query) add_sample(out, pix, color);  // else the inference kernel adds it
         rng_buf[pix] = rng;
     }
     const uint32_t slot = wave_append(query, query_count);
@@ -1081,34 +1126,6 @@ __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(const NrcRecord *__
 // One sample per pixel into the accumulation (pathtracer.wgsl:64-79); with
 // `post` (the render's last iteration) also the framebuffer
 // (pathtracer.wgsl:81-93) from the new accumulation.
-__global__ void nrc_accumulate_kernel(const float4 *__restrict__ partial, float4 *__restrict__ acc, uint32_t width,
-                                      uint32_t height, float cur, int post, float inv_gamma, uint32_t flip_y,
-                                      float4 *__restrict__ fb)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= width * height) return;
-    const float4 c = partial[i];
-    float4 a;
-    if (cur > 1.0f) {
-        a = acc[i];
-        a.x = a.x + c.x;
-        a.y = a.y + c.y;
-        a.z = a.z + c.z;
-        a.w = a.w + 1.0f;
-    } else {
-        a = make_float4(c.x, c.y, c.z, 1.0f);
-    }
-    acc[i] = a;
-    if (!post) return;
-    float4 o;
-    o.x = pow_f(fminf(fmaxf(a.x / cur, 0.0f), 1.0f), inv_gamma);
-    o.y = pow_f(fminf(fmaxf(a.y / cur, 0.0f), 1.0f), inv_gamma);
-    o.z = pow_f(fminf(fmaxf(a.z / cur, 0.0f), 1.0f), inv_gamma);
-    o.w = fminf(fmaxf(a.w / cur, 0.0f), 1.0f);
-    const uint32_t x = i % width, y = i / width;
-    fb[flip_y < 1u ? i : width * (height - y - 1u) + x] = o;
-}
-
 __global__ void nrc_reset_count_kernel(uint32_t *c)
 {
     if (threadIdx.x == 0) *c = 0u;
@@ -1126,9 +1143,13 @@ struct orn_nrc {
     NrcGrid grid;
     uint64_t grid_params;  // levels x table x 2
     float *mlp, *mlp_m, *mlp_v, *mlp_grad;
-    uint16_t *mlp_wt;
+    uint16_t *mlp_wt;  // working copies the kernels read
     float *grid_w, *grid_m, *grid_v, *grid_grad;
     __half *grid_half;
+    // spare working copies: orn_nrc_frame writes the next ones here beside
+    // the render, which reads mlp_wt / grid_half, and then swaps the pointers
+    uint16_t *mlp_wt_alt;
+    __half *grid_half_alt;
     float *partials;  // per training workgroup weight gradients
     uint32_t partial_groups;
     float4 *sample_u;   // per batch sample: unit position
@@ -1143,8 +1164,7 @@ struct orn_nrc {
     uint32_t *train_offsets;     // first record slot of each training-path workgroup
     uint32_t train_cap;
     NrcRecord *queries;
-    float4 *partial;
-    uint32_t query_cap;  // pixels the query and partial buffers hold
+    uint32_t query_cap;  // pixels the query buffer holds
     uint32_t *counters;  // [0] queries, [1] training records
     float *loss;         // [0] loss sum of the current step
     uint32_t step;       // Adam steps taken
@@ -1157,15 +1177,15 @@ struct orn_nrc {
 namespace {
 
 template <uint32_t F>
-void launch_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, uint32_t *rng, hipStream_t st,
-                  size_t lds)
+void launch_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, uint32_t *rng, const PixelSink &out,
+                  hipStream_t st, size_t lds)
 {
     const uint32_t tiles = blocks_for(p.width, 16u) * blocks_for(p.height, 16u);
     if (n->cfg.terminate_bounce == 0u)
-        nrc_primary_kernel<F><<<tiles, kBlock, (F & kHasMeshes) ? lds : 0, st>>>(ds, p, rng, n->partial, n->queries,
+        nrc_primary_kernel<F><<<tiles, kBlock, (F & kHasMeshes) ? lds : 0, st>>>(ds, p, rng, out, n->queries,
                                                                                 n->counters);
     else
-        nrc_paths_kernel<F><<<tiles, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, n->partial, n->queries,
+        nrc_paths_kernel<F><<<tiles, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, out, n->queries,
                                                         n->counters);
 }
 
@@ -1198,7 +1218,8 @@ void free_nrc(orn_nrc *n)
     (void)hipSetDevice(n->scene->ctx->device);
     (void)hipStreamSynchronize(n->scene->ctx->stream);
     void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
-                    n->grid_grad, n->grid_half, n->partials, n->train_recs, n->queries, n->partial, n->counters,
+                    n->grid_grad, n->grid_half, n->mlp_wt_alt, n->grid_half_alt, n->partials, n->train_recs,
+                    n->queries, n->counters,
                     n->loss, n->sample_u, n->sample_dx, n->sample_mask, n->grid_scratch, n->train_stage, n->train_keep,
                     n->train_block_keep, n->train_offsets};
     for (void *b : bufs)
@@ -1229,13 +1250,17 @@ AdamStep adam_step(const orn_nrc *n)
 }
 
 // Working copies (bf16 MLP, fp16 grid) from the masters.
-hipError_t refresh_params(orn_nrc *n, hipStream_t st)
+// Working copies (bf16 MLP, fp16 grid) from the masters, into wt and half
+// (by default the ones the kernels read).
+hipError_t refresh_params(orn_nrc *n, hipStream_t st, uint16_t *wt = nullptr, __half *half = nullptr)
 {
     const AdamStep a = adam_step(n);
     nrc_mlp_kernel<<<kMlpParams / 64u, 64u * kMlpSlices, 0, st>>>(nullptr, 0u, nullptr, 0u, 0u, a, n->mlp,
-                                                                 n->mlp_grad, n->mlp_m, n->mlp_v, n->mlp_wt, kRefresh);
-    const uint64_t g = n->grid_params;
-    nrc_grid_half_kernel<<<static_cast<uint32_t>((g + 255) / 256), 256, 0, st>>>(n->grid_w, n->grid_half, g);
+                                                                 n->mlp_grad, n->mlp_m, n->mlp_v,
+                                                                 wt ? wt : n->mlp_wt, kRefresh);
+    const uint64_t g4 = n->grid_params / 4u;  // grid_params = 2 x levels x 2^log2_table
+    nrc_grid_half_kernel<<<static_cast<uint32_t>((g4 + 255) / 256), 256, 0, st>>>(
+        reinterpret_cast<const float4 *>(n->grid_w), reinterpret_cast<uint2 *>(half ? half : n->grid_half), g4);
     return hipGetLastError();
 }
 
@@ -1390,6 +1415,7 @@ orn_status orn_nrc_create(orn_device_scene *scene, const orn_nrc_config *cfg, or
         (e = hipMalloc(&n->mlp_wt, kWtElems * 2)) || (e = hipMalloc(&n->grid_w, gp * 4)) ||
         (e = hipMalloc(&n->grid_m, gp * 4)) || (e = hipMalloc(&n->grid_v, gp * 4)) ||
         (e = hipMalloc(&n->grid_grad, gp * 4)) || (e = hipMalloc(&n->grid_half, gp * 2)) ||
+        (e = hipMalloc(&n->mlp_wt_alt, kWtElems * 2)) || (e = hipMalloc(&n->grid_half_alt, gp * 2)) ||
         (e = hipMalloc(&n->partials, size_t(n->partial_groups) * kMlpParams * 4)) ||
         (e = hipMalloc(&n->train_recs, size_t(n->train_cap) * sizeof(NrcRecord))) ||
         (e = hipMalloc(&n->train_stage, size_t(std::max(cfg->train_paths, 1u)) * sizeof(NrcRecord))) ||
@@ -1503,7 +1529,7 @@ orn_status orn_nrc_query(orn_nrc *n, const orn_nrc_record *records, size_t count
         const uint32_t tiles = blocks_for(static_cast<uint32_t>(count), kTileRows);
         nrc_infer_kernel<<<std::min<uint32_t>(tiles, 2048u), kBlock, 0, st>>>(
             n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, d_rec, nullptr,
-            static_cast<uint32_t>(count), 0, d_out, nullptr, d_in);
+            static_cast<uint32_t>(count), 0, d_out, PixelSink{}, d_in);
         e = hipGetLastError();
     }
     if (!e) e = hipMemcpyAsync(rgb, d_out, count * 12, hipMemcpyDeviceToHost, st);
@@ -1568,15 +1594,12 @@ orn_status enqueue_render(orn_nrc *n, orn_target *t, const orn_constant_params *
     orn_device_scene *s = n->scene;
     hipStream_t st = s->ctx->stream;
     const uint32_t px = t->width * t->height;
-    if (px > n->query_cap) {  // query and per-pixel sample buffers grow with the target
+    if (px > n->query_cap) {  // the query buffer grows with the target
         ORN_TRY(hipStreamSynchronize(st));
         if (n->queries) (void)hipFree(n->queries);
-        if (n->partial) (void)hipFree(n->partial);
         n->queries = nullptr;
-        n->partial = nullptr;
         n->query_cap = 0;
         ORN_TRY(hipMalloc(&n->queries, size_t(px) * sizeof(NrcRecord)));
-        ORN_TRY(hipMalloc(&n->partial, size_t(px) * sizeof(float4)));
         n->query_cap = px;
     }
     DevScene ds = s->ds;
@@ -1585,16 +1608,24 @@ orn_status enqueue_render(orn_nrc *n, orn_target *t, const orn_constant_params *
     float cur = params->current_iteration;
     const uint32_t iters = std::max(iterations, 1u);
     for (uint32_t it = 0; it < iters; it++) {
+        // samples that end without a query are added by the path kernel,
+        // the others by the inference kernel
+        cur = cur + 1.0f;
+        PixelSink out;
+        out.acc = t->acc;
+        out.fb = t->fb;
+        out.width = t->width;
+        out.height = t->height;
+        out.cur = cur;
+        out.post = it + 1u == iters ? 1 : 0;
+        out.inv_gamma = params->inverted_gamma;
+        out.flip_y = params->flip_y;
         nrc_reset_count_kernel<<<1, 64, 0, st>>>(n->counters);
-        ORN_NRC_DISPATCH(launch_paths, s->features, n, ds, *params, t->rng, st, lds);
+        ORN_NRC_DISPATCH(launch_paths, s->features, n, ds, *params, t->rng, out, st, lds);
         const uint32_t tiles = blocks_for(px, kTileRows);
         nrc_infer_kernel<<<std::min<uint32_t>(tiles, uint32_t(s->ctx->compute_units) * 2u), kBlock, 0, st>>>(
             n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, n->queries, n->counters, px, 1,
-            nullptr, n->partial, nullptr);
-        cur = cur + 1.0f;
-        nrc_accumulate_kernel<<<blocks_for(px, 256), 256, 0, st>>>(n->partial, t->acc, t->width, t->height, cur,
-                                                                   it + 1u == iters ? 1 : 0, params->inverted_gamma,
-                                                                   params->flip_y, t->fb);
+            nullptr, out, nullptr);
     }
     ORN_TRY(hipGetLastError());
     return ORN_OK;
@@ -1613,22 +1644,19 @@ orn_status enqueue_train_paths(orn_nrc *n, const orn_constant_params *params, ui
     return ORN_OK;
 }
 
-// `steps` Adam steps on the frame's training records, on the scene's stream.
-// first_trained: the first step's train_gradients (and the loss reset) have
-// been enqueued already (orn_nrc_frame runs them beside the render).
-orn_status enqueue_steps(orn_nrc *n, uint32_t steps, bool first_trained = false)
+// Adam steps first..steps-1 on the frame's training records, on stream st
+// (first = 0 also resets the loss sum). Step k takes records k*b, k*b + 1,
+// ... of this frame's samples, modulo their count, min(b, count) of them (an
+// empty frame makes a step with zero gradient). kUpdateMasters leaves the
+// working copies to a later refresh_params and suits only a single step.
+orn_status enqueue_steps(orn_nrc *n, uint32_t first, uint32_t steps, GradMode mode, hipStream_t st)
 {
-    hipStream_t st = n->scene->ctx->stream;
-    if (!first_trained) nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
+    if (first == 0u && steps > 0u) nrc_zero_kernel<<<1, 32, 0, st>>>(n->loss, 4);
     const uint32_t b = n->cfg.train_batch;
-    for (uint32_t k = 0; k < steps; k++) {
-        // step k takes records k*b, k*b + 1, ... of this frame's samples,
-        // modulo their count, min(b, count) of them (an empty frame makes a
-        // step with zero gradient)
+    for (uint32_t k = first; k < steps; k++) {
         n->step++;
-        if (k > 0u || !first_trained)
-            ORN_TRY(train_gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
-        ORN_TRY(apply_gradients(n, n->counters + 1, n->train_cap, b, kUpdate, st));
+        ORN_TRY(train_gradients(n, n->train_recs, n->counters + 1, n->train_cap, k * b, b, st));
+        ORN_TRY(apply_gradients(n, n->counters + 1, n->train_cap, b, mode, st));
     }
     return ORN_OK;
 }
@@ -1668,7 +1696,7 @@ orn_status orn_nrc_train(orn_nrc *n, const orn_constant_params *params, uint32_t
     hipStream_t st = n->scene->ctx->stream;
     ORN_TRY(hipEventRecord(n->ev[2], st));
     orn_status r = enqueue_train_paths(n, params, frame, st);
-    if (r == ORN_OK) r = enqueue_steps(n, steps);
+    if (r == ORN_OK) r = enqueue_steps(n, 0u, steps, kUpdate, st);
     if (r != ORN_OK) return r;
     ORN_TRY(hipEventRecord(n->ev[3], st));
     n->trained = true;
@@ -1690,18 +1718,28 @@ orn_status orn_nrc_frame(orn_nrc *n, orn_target *t, const orn_constant_params *p
     hipStream_t st = n->scene->ctx->stream;
     ORN_TRY(hipEventRecord(n->frame_start, st));
     ORN_TRY(hipEventRecord(n->fork, st));
+    // the render first, so that its long path kernel starts at once
+    if ((r = enqueue_render(n, t, params, iterations)) != ORN_OK) return r;
     ORN_TRY(hipStreamWaitEvent(n->side, n->fork, 0));
     if ((r = enqueue_train_paths(n, params, frame, n->side)) != ORN_OK) return r;
-    // the first step's forward + backward only reads the network, so it runs
-    // beside the render too; the kernels that write the network wait for both
+    // The render reads only the working copies (bf16 MLP, fp16 grid). The
+    // first step -- forward, backward and the Adam step on the fp32 masters
+    // and moments -- therefore runs beside it, and so does the rewrite of the
+    // spare working copies from the updated masters (what the step's own
+    // kUpdate would leave in them, since each working copy is its rounded
+    // master); once both streams are done the spare copies become current,
+    // and any further steps follow on the scene's stream.
     if (steps > 0u) {
-        nrc_zero_kernel<<<1, 32, 0, n->side>>>(n->loss, 4);
-        ORN_TRY(train_gradients(n, n->train_recs, n->counters + 1, n->train_cap, 0u, n->cfg.train_batch, n->side));
+        if ((r = enqueue_steps(n, 0u, 1u, kUpdateMasters, n->side)) != ORN_OK) return r;
+        ORN_TRY(refresh_params(n, n->side, n->mlp_wt_alt, n->grid_half_alt));
     }
     ORN_TRY(hipEventRecord(n->join, n->side));
-    if ((r = enqueue_render(n, t, params, iterations)) != ORN_OK) return r;
-    ORN_TRY(hipStreamWaitEvent(st, n->join, 0));  // the updates read the gradients; the render read the weights
-    if ((r = enqueue_steps(n, steps, steps > 0u)) != ORN_OK) return r;
+    ORN_TRY(hipStreamWaitEvent(st, n->join, 0));
+    if (steps > 0u) {
+        std::swap(n->mlp_wt, n->mlp_wt_alt);
+        std::swap(n->grid_half, n->grid_half_alt);
+    }
+    if ((r = enqueue_steps(n, 1u, steps, kUpdate, st)) != ORN_OK) return r;
     ORN_TRY(hipEventRecord(n->frame_stop, st));
     n->framed = true;
     if (loss) {
