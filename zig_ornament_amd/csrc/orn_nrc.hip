@@ -366,39 +366,114 @@ __device__ __forceinline__ void add_sample(const PixelSink &o, uint32_t i, f3 c)
     o.fb[o.flip_y < 1u ? i : o.width * (o.height - y - 1u) + x] = v;
 }
 
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+// The forward of one layer for the 16-sample tile in rows X[0..15], in
+// place: every lane first loads its sample's inputs (lane (r, q) the K
+// slices 8q..8q+7 and 32+8q..32+8q+7 of row r), then the layer's four
+// 16-wide output tiles are formed from the transposed weights Wt ([n][k]
+// rows) and stored, relu'd and rounded to bf16, over the same row. The
+// wave's loads are issued before any of its stores, so no lane reads an
+// output. With out (the 16-wide output layer) the FP32 tile is returned
+// instead and nothing is stored.
+__device__ __forceinline__ void rows_forward(const uint16_t *Wt, uint32_t n_tiles, uint16_t *X, uint32_t lane,
+                                             f32x4 *out)
+{
+    const uint32_t r = lane % 16u, q = lane / 16u;
+    const bf16x8 b0 = row8(X + r * kStride + 8u * q);
+    const bf16x8 b1 = row8(X + r * kStride + 32u + 8u * q);
+    f32x4 acc[4];
+    for (uint32_t nt = 0; nt < n_tiles; nt++) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = mfma32(row8(Wt + (nt * 16u + r) * kStride + 8u * q), b0, c);
+        c = mfma32(row8(Wt + (nt * 16u + r) * kStride + 32u + 8u * q), b1, c);
+        acc[nt] = c;  // acc[nt][j] = Y[r][nt*16 + 4q + j]
+    }
+    if (out) {
+        *out = acc[0];
+        return;
+    }
+    for (uint32_t nt = 0; nt < n_tiles; nt++) {
+        uint16_t h[4];
+        for (int j = 0; j < 4; j++) h[j] = to_bf16(fmaxf(acc[nt][j], 0.0f));
+        uint2 v;
+        v.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
+        v.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+        *reinterpret_cast<uint2 *>(X + r * kStride + nt * 16u + 4u * q) = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Inference: predictions for n records, 64 per workgroup tile (persistent).
-// mode 0: raw predictions to out_rgb[3 i]; mode 1: the sample of the
-// record's pixel, weight * max(prediction, 0), into `out` (a path that
-// reaches its query has gathered no light before it: emitters end paths).
+// Inference: predictions for n records. The workgroup stages the weights in
+// LDS once; after that every wave works alone, on 16-record tiles (tile
+// index blockIdx.x * kInferWaves + wave, then strides of the whole grid), in
+// its own 16 LDS rows: encoding (4 lanes per record) and the six layers,
+// in place, with no workgroup barrier. mode 0: raw predictions to
+// out_rgb[3 i]; mode 1: the sample of the record's pixel, weight *
+// max(prediction, 0), into `out` (a path that reaches its query has gathered
+// no light before it: emitters end paths).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __half2 *__restrict__ table,
-                                                           const uint16_t *__restrict__ wt,
-                                                           const NrcRecord *__restrict__ recs,
-                                                           const uint32_t *__restrict__ count_ptr, uint32_t count_max,
-                                                           int mode, float *__restrict__ out_rgb, PixelSink out,
-                                                           float *__restrict__ inputs)
+constexpr uint32_t kInferWaves = 16;
+__global__ __launch_bounds__(64u * kInferWaves) void nrc_infer_kernel(NrcGrid g, const __half2 *__restrict__ table,
+                                                                      const uint16_t *__restrict__ wt,
+                                                                      const NrcRecord *__restrict__ recs,
+                                                                      const uint32_t *__restrict__ count_ptr,
+                                                                      uint32_t count_max, int mode,
+                                                                      float *__restrict__ out_rgb, PixelSink out,
+                                                                      float *__restrict__ inputs)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_wt[kWtElems];
-    __shared__ __attribute__((aligned(16))) uint16_t s_x[2][kTileRows * kStride];
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][16u * kStride];
     const uint32_t n = min(count_ptr ? *count_ptr : count_max, count_max);
     const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
-    load_weights(wt, s_wt);
-    for (uint32_t tile = blockIdx.x; tile * kTileRows < n; tile += gridDim.x) {
-        const uint32_t first = tile * kTileRows;
-        __syncthreads();
-        encode_tile(g, table, recs, first, min(n - first, kTileRows), 0u, s_x[0], inputs);
-        __syncthreads();
-        uint32_t cur = 0;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(wt);
+        uint4 *dst = reinterpret_cast<uint4 *>(s_wt);
+        for (uint32_t i = threadIdx.x; i < kWtElems / 8u; i += 64u * kInferWaves) dst[i] = src[i];
+    }
+    __syncthreads();
+    uint16_t *X = s_x[wave];
+    for (uint32_t tile = blockIdx.x * kInferWaves + wave; tile * 16u < n; tile += gridDim.x * kInferWaves) {
+        const uint32_t first = tile * 16u;
+        {  // encoding: lane group s = lane / 4 holds record first + s, 4 levels per lane
+            const uint32_t sr = lane / 4u, part = lane % 4u;
+            uint16_t *row = X + sr * kStride;
+            if (first + sr >= n) {
+                for (uint32_t k = part * 16u; k < part * 16u + 16u; k++) row[k] = 0u;
+            } else {
+                const uint32_t i = first + sr;
+                const NrcRecord &rc = recs[i];
+                float u[3];
+                unit_position(g, rc.pos, u);
+                for (uint32_t l = part * 4u; l < part * 4u + 4u; l++) {
+                    float f0, f1;
+                    level_features(g, table, l, u, f0, f1);
+                    row[2u * l] = to_bf16(f0);
+                    row[2u * l + 1u] = to_bf16(f1);
+                    if (inputs) {
+                        inputs[size_t(i) * kIn + 2u * l] = f0;
+                        inputs[size_t(i) * kIn + 2u * l + 1u] = f1;
+                    }
+                }
+                for (uint32_t k = kGridFeatures + part * 8u; k < kGridFeatures + part * 8u + 8u; k++) {
+                    const float v = extra_input(rc, k);
+                    row[k] = to_bf16(v);
+                    if (inputs) inputs[size_t(i) * kIn + k] = v;
+                }
+            }
+        }
+        // The rows are this wave's alone: its LDS accesses complete in the
+        // order it issues them, so only the compiler must keep that order (a
+        // wavefront-scope fence emits no instruction).
+        wave_lds_order();
         for (uint32_t l = 0; l < kHidden; l++) {
-            layer_forward(layer_wt(s_wt, l), kW / 16u, s_x[cur], s_x[cur ^ 1u], wave, lane, nullptr);
-            cur ^= 1u;
-            __syncthreads();
+            rows_forward(layer_wt(s_wt, l), kW / 16u, X, lane, nullptr);
+            wave_lds_order();
         }
         f32x4 y;
-        layer_forward(layer_wt(s_wt, kHidden), 1u, s_x[cur], nullptr, wave, lane, &y);
-        // lanes 0..15 hold channels 0..3 of sample wave*16 + lane
-        const uint32_t i = first + wave * 16u + lane;
+        rows_forward(layer_wt(s_wt, kHidden), 1u, X, lane, &y);
+        // lanes 0..15 hold channels 0..3 of record first + lane
+        const uint32_t i = first + lane;
         if (lane < 16u && i < n) {
             if (mode == 0) {
                 out_rgb[3u * i] = y[0];
@@ -411,6 +486,7 @@ __global__ __launch_bounds__(kBlock) void nrc_infer_kernel(NrcGrid g, const __ha
                               rc.weight[2] * fmaxf(y[2], 0.0f)));
             }
         }
+        wave_lds_order();  // the next tile's encoding overwrites the rows
     }
 }
 
@@ -1527,7 +1603,7 @@ orn_status orn_nrc_query(orn_nrc *n, const orn_nrc_record *records, size_t count
     if (!e) e = hipMemcpyAsync(d_rec, records, count * sizeof(NrcRecord), hipMemcpyHostToDevice, st);
     if (!e) {
         const uint32_t tiles = blocks_for(static_cast<uint32_t>(count), kTileRows);
-        nrc_infer_kernel<<<std::min<uint32_t>(tiles, 2048u), kBlock, 0, st>>>(
+        nrc_infer_kernel<<<std::min<uint32_t>(blocks_for(tiles * 4u, kInferWaves), 1024u), 64u * kInferWaves, 0, st>>>(
             n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, d_rec, nullptr,
             static_cast<uint32_t>(count), 0, d_out, PixelSink{}, d_in);
         e = hipGetLastError();
@@ -1623,7 +1699,8 @@ orn_status enqueue_render(orn_nrc *n, orn_target *t, const orn_constant_params *
         nrc_reset_count_kernel<<<1, 64, 0, st>>>(n->counters);
         ORN_NRC_DISPATCH(launch_paths, s->features, n, ds, *params, t->rng, out, st, lds);
         const uint32_t tiles = blocks_for(px, kTileRows);
-        nrc_infer_kernel<<<std::min<uint32_t>(tiles, uint32_t(s->ctx->compute_units) * 2u), kBlock, 0, st>>>(
+        nrc_infer_kernel<<<std::min<uint32_t>(blocks_for(tiles * 4u, kInferWaves), uint32_t(s->ctx->compute_units)),
+                           64u * kInferWaves, 0, st>>>(
             n->grid, reinterpret_cast<const __half2 *>(n->grid_half), n->mlp_wt, n->queries, n->counters, px, 1,
             nullptr, out, nullptr);
     }
