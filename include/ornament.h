@@ -109,6 +109,15 @@ orn_status ornament_camera_get_dirty(const ornament_scene *s, int *dirty);
  * material_index / texture_id / bvh_id on the scene's objects. */
 orn_status ornament_bvh_init(ornament_scene *s, int row_major_transforms, ornament_bvh **out);
 void ornament_bvh_deinit(ornament_bvh *bvh);
+/* The generator of the builds' random split axes (bvh.zig:12-13: one
+ * DefaultPrng per process, seeded with 1244, and bvh.zig:239, 331). The
+ * reference shares it across every Bvh.init of the process, so a build
+ * depends on the builds before it. mode 1 does the same: the builds that
+ * follow continue one stream (one build at a time), which this call restarts
+ * from seed 1244, as a new process would. mode 0 (the default) seeds each
+ * build with 1244: every build is then the one the reference makes first in
+ * a process. Other modes: InvalidValue. */
+orn_status ornament_bvh_shared_generator(int mode);
 /* Pointers into the Bvh's arrays (valid until ornament_bvh_deinit); textures
  * point into the scene's texture data. */
 orn_status ornament_bvh_arrays(const ornament_bvh *bvh, orn_scene_desc *out);

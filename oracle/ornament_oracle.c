@@ -1902,6 +1902,13 @@ static orc_bvh_node build_tlas(bb_ctx *b, bb_leaf *l, uint32_t n)
 
 int orc_bvh_build(const orc_build_input *in, orc_build_output *out)
 {
+    uint64_t prng[4];
+    orc_xoshiro_seed(1244u, prng); /* bvh.zig:12 DefaultPrng.init(1244) */
+    return orc_bvh_build_stream(in, out, prng);
+}
+
+int orc_bvh_build_stream(const orc_build_input *in, orc_build_output *out, uint64_t prng[4])
+{
     memset(out, 0, sizeof(*out));
     uint32_t shapes = in->spheres_len + in->meshes_len + in->instances_len;
     if (shapes == 0u) return -1; /* bvh.zig:35-37 panics on an empty scene */
@@ -1910,7 +1917,7 @@ int orc_bvh_build(const orc_build_input *in, orc_build_output *out)
     bb_ctx b;
     memset(&b, 0, sizeof(b));
     b.in = in;
-    orc_xoshiro_seed(1244u, b.prng); /* bvh.zig:12 DefaultPrng.init(1244) */
+    memcpy(b.prng, prng, sizeof(b.prng)); /* the process's generator (bvh.zig:12-13) */
     b.tlas.elem = b.blas.elem = sizeof(orc_bvh_node);
     b.normals.elem = 16;
     b.normal_indices.elem = b.uv_indices.elem = b.textures.elem = 4;
@@ -1953,6 +1960,7 @@ int orc_bvh_build(const orc_build_input *in, orc_build_output *out)
     out->materials_len = b.materials.len;
     out->textures = (uint32_t *)b.textures.p;
     out->textures_len = b.textures.len;
+    memcpy(prng, b.prng, sizeof(b.prng)); /* the next build continues the stream */
     return 0;
 }
 

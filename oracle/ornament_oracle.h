@@ -321,6 +321,11 @@ typedef struct orc_build_output {
 } orc_build_output;
 
 int orc_bvh_build(const orc_build_input *in, orc_build_output *out);
+/* The same build drawing its split axes from the caller's Xoshiro256 state,
+ * which it leaves advanced: the reference's one generator per process
+ * (bvh.zig:12-13) across consecutive builds. orc_bvh_build is this with a
+ * state freshly seeded with 1244. */
+int orc_bvh_build_stream(const orc_build_input *in, orc_build_output *out, uint64_t prng[4]);
 void orc_bvh_free(orc_build_output *out);
 
 /* Zig 0.11 std.rand.Xoshiro256 seeded through SplitMix64, and

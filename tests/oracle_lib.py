@@ -151,6 +151,7 @@ def lib():
             "orc_plane_mesh_transform": (None, [P(f32), f32, f32, P(f32), P(f32)]),
             "orc_mesh_aabb": (None, [vp, vp, u32, P(f32), P(f32), P(f32), P(f32), P(f32)]),
             "orc_bvh_build": (C.c_int, [P(BuildInput), P(BuildOutput)]),
+            "orc_bvh_build_stream": (C.c_int, [P(BuildInput), P(BuildOutput), P(C.c_uint64)]),
             "orc_bvh_free": (None, [P(BuildOutput)]),
             "orc_splitmix64_next": (C.c_uint64, [P(C.c_uint64)]),
             "orc_xoshiro_seed": (None, [C.c_uint64, P(C.c_uint64)]),
@@ -300,8 +301,11 @@ PLANE_NORMALS = np.array([[0, 1, 0, 0]] * 4, dtype=np.float32)
 PLANE_UVS = np.full((4, 2), 0.5, dtype=np.float32)
 
 
-def build_bvh(spec):
-    """Oracle restatement of Scene factories + Bvh.init over a SceneSpec."""
+def build_bvh(spec, prng=None):
+    """Oracle restatement of Scene factories + Bvh.init over a SceneSpec.
+    prng: a (c_uint64 * 4) Xoshiro256 state to draw the split axes from and
+    leave advanced (one generator across builds, bvh.zig:12-13); None seeds
+    the build with 1244."""
     L = lib()
     keep = []
     mats = (BuildMaterial * max(1, len(spec.materials)))()
@@ -365,7 +369,10 @@ def build_bvh(spec):
     bi_.meshes, bi_.meshes_len = C.addressof(meshes), len(spec.meshes)
     bi_.instances, bi_.instances_len = C.addressof(inst), len(spec.instances)
     out = BuildOutput()
-    rc = L.orc_bvh_build(C.byref(bi_), C.byref(out))
+    if prng is None:
+        rc = L.orc_bvh_build(C.byref(bi_), C.byref(out))
+    else:
+        rc = L.orc_bvh_build_stream(C.byref(bi_), C.byref(out), prng)
     if rc != 0:
         raise RuntimeError(f"orc_bvh_build failed: {rc}")
     try:

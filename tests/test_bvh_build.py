@@ -158,3 +158,31 @@ def test_empty_scene_is_an_error():
     with pytest.raises(zo.OrnamentError) as e:
         zo.Bvh(s)
     assert e.value.status == 9001
+
+
+def test_shared_generator_builds_follow_the_reference_stream():
+    """bvh.zig:12-13 keeps one split-axis generator per process, seeded once
+    with 1244, so each build continues where the previous one stopped. With
+    ornament_bvh_shared_generator(1), three builds in a row (C2, C1, C2)
+    equal the oracle's builds drawing from one Xoshiro256 state seeded once;
+    the second C2 build differs from the first (the stream moved on), and
+    mode 0 gives back the per-build seeding."""
+    sp = specs()
+    order = ("C2", "C1", "C2")
+    state = (C.c_uint64 * 4)()
+    O.lib().orc_xoshiro_seed(1244, state)
+    zo.bvh_shared_generator(True)
+    try:
+        got = [zo.Bvh(W.build_scene(sp[n])) for n in order]
+    finally:
+        zo.bvh_shared_generator(False)
+    want = [O.build_bvh(sp[n], prng=state) for n in order]
+    for g, w in zip(got, want):
+        for f in FIELDS:
+            assert getattr(g, f).tobytes() == getattr(w, f).tobytes(), f
+    assert got[0].tlas_nodes.tobytes() != got[2].tlas_nodes.tobytes()
+    fresh = zo.Bvh(W.build_scene(sp["C2"]))
+    assert fresh.tlas_nodes.tobytes() == got[0].tlas_nodes.tobytes()
+    assert fresh.tlas_nodes.tobytes() == O.build_bvh(sp["C2"]).tlas_nodes.tobytes()
+    assert zo.lib().ornament_bvh_shared_generator(2) == 1  # ORN_ERROR_INVALID_VALUE, mode unchanged
+    assert zo.Bvh(W.build_scene(sp["C2"])).tlas_nodes.tobytes() == got[0].tlas_nodes.tobytes()

@@ -225,6 +225,13 @@ def _array(ptr, n, dtype, shape_tail=()):
     return np.frombuffer(bytes(buf), dtype=dtype).reshape((int(n),) + tuple(shape_tail)).copy()
 
 
+def bvh_shared_generator(shared):
+    """ornament_bvh_shared_generator: True makes the builds that follow share
+    one split-axis generator, restarted from seed 1244, as the reference's
+    process-wide one (bvh.zig:12-13); False (the default) seeds every build."""
+    check(lib().ornament_bvh_shared_generator(1 if shared else 0), "ornament_bvh_shared_generator")
+
+
 class Bvh:
     """Bvh.init (bvh.zig:33-78): builds and copies out the device arrays."""
 

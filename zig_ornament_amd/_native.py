@@ -197,6 +197,7 @@ HOST_API = [
     ("ornament_camera_get", st, [vp, P(Camera)]),
     ("ornament_camera_get_dirty", st, [vp, P(C.c_int)]),
     ("ornament_bvh_init", st, [vp, C.c_int, P(vp)]),
+    ("ornament_bvh_shared_generator", st, [C.c_int]),
     ("ornament_bvh_deinit", None, [vp]),
     ("ornament_bvh_arrays", st, [vp, P(SceneDesc)]),
     ("ornament_bvh_texture_sources", st, [vp, up, sz]),

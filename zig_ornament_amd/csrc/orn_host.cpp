@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <unordered_set>
 #include <vector>
@@ -314,8 +315,10 @@ namespace {
 
 // std.rand.DefaultPrng of Zig 0.11: Xoshiro256 (xoshiro256++) seeded by
 // SplitMix64. The reference shares one generator seeded with 1244 across all
-// builds in a process (bvh.zig:12-13); this layer reseeds it per Bvh.init so
-// a build does not depend on what was built before it in the same process.
+// builds in a process (bvh.zig:12-13). By default this layer reseeds it per
+// Bvh.init, so that a build does not depend on what was built before it in
+// the same process; ornament_bvh_shared_generator(1) switches to one shared
+// stream, as in the reference.
 struct Xoshiro256 {
     uint64_t s[4];
     explicit Xoshiro256(uint64_t seed)
@@ -429,6 +432,14 @@ Aabb bounding_box(It begin, It end, BoxOf box)
     }
     return b;
 }
+
+// The process's build generator when builds share one stream (see Xoshiro256).
+struct SharedBvhPrng {
+    std::mutex mutex;
+    bool shared = false;
+    uint64_t state[4] = {0, 0, 0, 0};
+};
+SharedBvhPrng g_bvh_prng;
 
 }  // namespace
 
@@ -1042,9 +1053,27 @@ orn_status ornament_bvh_init(ornament_scene *s, int row_major, ornament_bvh **ou
     *out = nullptr;
     auto b = std::make_unique<ornament_bvh>();
     b->row_major_transforms = row_major != 0;
+    // with the shared stream, builds run one at a time and each takes the
+    // generator where the previous one left it
+    std::unique_lock<std::mutex> lock(g_bvh_prng.mutex);
+    if (g_bvh_prng.shared)
+        std::memcpy(b->rand.s, g_bvh_prng.state, sizeof(b->rand.s));
+    else
+        lock.unlock();
     orn_status st = b->build(s);
+    if (lock.owns_lock()) std::memcpy(g_bvh_prng.state, b->rand.s, sizeof(b->rand.s));
     if (st != ORN_OK) return st;
     *out = b.release();
+    return ORN_OK;
+}
+
+orn_status ornament_bvh_shared_generator(int mode)
+{
+    if (mode != 0 && mode != 1) return ORN_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> lock(g_bvh_prng.mutex);
+    g_bvh_prng.shared = mode == 1;
+    const Xoshiro256 fresh{1244u};  // bvh.zig:12: DefaultPrng.init(1244)
+    std::memcpy(g_bvh_prng.state, fresh.s, sizeof(g_bvh_prng.state));
     return ORN_OK;
 }
 
