@@ -231,6 +231,27 @@ def test_primary_queries_trace_like_the_plain_render(device, name):
     assert same_bits(a.framebuffer(), b.framebuffer())
 
 
+def test_render_iterations_accumulate_like_single_calls(device):
+    """One orn_nrc_render of 3 iterations equals three 1-iteration renders
+    with current_iteration 0, 1 and 2: the samples the inference kernel adds
+    (queries) and those the path kernel adds (sky, emitters) accumulate as
+    State.nextIteration does, and the framebuffer is post-processed once,
+    from the 3-sample sum."""
+    spec, ds = scene(device, seed=41)
+    nrc = Nrc(ds, log2_table=12, terminate_bounce=0, train_paths=0)
+    random_params(nrc, 5)
+    a = zo.Target(ds, spec.width, spec.height, 3)
+    b = zo.Target(ds, spec.width, spec.height, 3)
+    nrc.render(a, zo_params(spec), 3)
+    for it in range(3):
+        nrc.render(b, zo_params(spec, float(it)), 1)
+    acc = a.accumulation()
+    assert np.all(acc[:, 3] == 3.0)
+    assert same_bits(acc, b.accumulation())
+    assert np.array_equal(a.rng_state(), b.rng_state())
+    assert same_bits(a.framebuffer(), b.framebuffer())
+
+
 def rmse(img, ref):
     return float(np.sqrt(np.mean((img[:, :3].astype(np.float64) - ref[:, :3]) ** 2)))
 
