@@ -482,7 +482,9 @@ orn_status orn_trace_rays_packet(orn_device_scene *scene, float t_min, const flo
 
 /* Evaluates the device's FP32 transcendental functions (the kernels' stand-ins
  * for the WGSL builtins pow, sin, cos, acos, atan2) over host arrays:
- * op 0 sin(a), 1 cos(a), 2 pow(a, b), 3 acos(a), 4 atan2(a, b). */
+ * op 0 sin(a), 1 cos(a), 2 pow(a, b), 3 acos(a), 4 atan2(a, b); and op 5 the
+ * radiance cache's hardware bf16 rounding of an activation, returned as the
+ * float the rounded value stands for. */
 orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, float *out,
                          size_t n);
 

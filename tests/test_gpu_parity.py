@@ -6,6 +6,9 @@ floating-point operation identically); relMSE <= 1e-4 is asserted as well.
 Closest hits are also checked against the FP64 brute-force oracle: equal IDs
 on every ray it does not flag as an edge/tie case, t within 1e-5 relative.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -153,6 +156,30 @@ def test_image_matches_oracle(device, name, tuning):
     assert exact.mean() == 1.0
     assert np.array_equal(rng_g, rng_o)
     assert same_bits(fb_g, fb_o)
+
+
+def test_activation_bf16_rounding_is_round_to_nearest_even(device):
+    """The radiance cache's MLP rounds its relu'd activations to bf16 with
+    the hardware conversion (orn_eval_math op 5). For every non-NaN float it
+    must equal the oracle's round to nearest even, (u + 0x7fff + ((u >> 16)
+    & 1)) >> 16: random bit patterns of every exponent, exact ties with an
+    even and an odd kept half, the values next to ties, subnormals, +-0,
+    +-inf and the largest floats (which round up to inf)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import nrc_oracle as R
+    rng = np.random.default_rng(17)
+    bits = rng.integers(0, 1 << 32, 1 << 18, dtype=np.uint64).astype(np.uint32)
+    hi = rng.integers(0, 1 << 16, 4096, dtype=np.uint64).astype(np.uint32) << 16
+    ties = np.concatenate([hi | 0x8000, hi | 0x7FFF, hi | 0x8001])
+    special = np.array([0, 0x80000000, 1, 0x7FFFFF, 0x80000001, 0x807FFFFF, 0x7F800000, 0xFF800000,
+                        0x7F7FFFFF, 0xFF7FFFFF, 0x7F7F8000, 0x7F7E8000, 0x00008000, 0x00018000],
+                       np.uint32)
+    u = np.concatenate([bits, ties, special])
+    a = u.view(np.float32)
+    a = a[~np.isnan(a)]
+    g = device.eval_math(5, a)
+    o = R.bf16(a)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), a[g.view(np.uint32) != o.view(np.uint32)][:8]
 
 
 def test_math_functions_bit_exact(device):

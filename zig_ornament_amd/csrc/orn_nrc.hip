@@ -99,13 +99,16 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-__device__ __forceinline__ uint32_t grid_index(const NrcGrid &g, uint32_t l, uint32_t cx, uint32_t cy, uint32_t cz)
+// Table entry of lattice point (cx, cy, cz) at a level of `res` cells per
+// axis: direct (dense) or hashed with the Instant-NGP primes.
+__device__ __forceinline__ uint32_t grid_index_at(uint32_t res, uint32_t dense, uint32_t log2_table, uint32_t cx,
+                                                  uint32_t cy, uint32_t cz)
 {
-    if (g.dense[l]) {
-        const uint32_t r1 = g.res[l] + 1u;
+    if (dense) {
+        const uint32_t r1 = res + 1u;
         return cx + r1 * (cy + r1 * cz);
     }
-    return (cx ^ (cy * 2654435761u) ^ (cz * 805459861u)) & ((1u << g.log2_table) - 1u);
+    return (cx ^ (cy * 2654435761u) ^ (cz * 805459861u)) & ((1u << log2_table) - 1u);
 }
 
 // scene position -> [0, 1]^3
@@ -114,46 +117,99 @@ __device__ __forceinline__ void unit_position(const NrcGrid &g, const float pos[
     for (int a = 0; a < 3; a++) u[a] = fminf(fmaxf((pos[a] - g.lo[a]) * g.inv_ext[a], 0.0f), 1.0f);
 }
 
-// The 8 corners of the cell around u at level l, with trilinear weights
-// ((wx * wy) * wz, corner bit 0 = x, bit 1 = y, bit 2 = z).
+// The cell around u at a level of `res` cells per axis: its lower lattice
+// corner c0 and the fractional position fr inside it.
+__device__ __forceinline__ void cell_at(uint32_t res, const float u[3], uint32_t c0[3], float fr[3])
+{
+    const float r = static_cast<float>(res);
+    for (int a = 0; a < 3; a++) {
+        const float x = u[a] * r;
+        const uint32_t c = min(static_cast<uint32_t>(floorf(x)), res - 1u);
+        c0[a] = c;
+        fr[a] = x - static_cast<float>(c);
+    }
+}
+
+// Trilinear weight of corner c (bit 0 = x, bit 1 = y, bit 2 = z): (wx * wy) * wz.
+__device__ __forceinline__ float corner_weight(const float fr[3], uint32_t c)
+{
+    const uint32_t bx = c & 1u, by = (c >> 1) & 1u, bz = (c >> 2) & 1u;
+    const float wx = bx ? fr[0] : 1.0f - fr[0];
+    const float wy = by ? fr[1] : 1.0f - fr[1];
+    const float wz = bz ? fr[2] : 1.0f - fr[2];
+    return (wx * wy) * wz;
+}
+
+// The 8 corners of the cell around u at level l, with their trilinear weights.
 __device__ __forceinline__ void level_corners(const NrcGrid &g, uint32_t l, const float u[3], uint32_t idx[8],
                                               float w[8])
 {
     uint32_t c0[3];
     float fr[3];
-    const float r = static_cast<float>(g.res[l]);
-    for (int a = 0; a < 3; a++) {
-        const float x = u[a] * r;
-        const uint32_t c = min(static_cast<uint32_t>(floorf(x)), g.res[l] - 1u);
-        c0[a] = c;
-        fr[a] = x - static_cast<float>(c);
-    }
+    cell_at(g.res[l], u, c0, fr);
     for (uint32_t c = 0; c < 8u; c++) {
-        const uint32_t bx = c & 1u, by = (c >> 1) & 1u, bz = (c >> 2) & 1u;
-        idx[c] = grid_index(g, l, c0[0] + bx, c0[1] + by, c0[2] + bz);
-        const float wx = bx ? fr[0] : 1.0f - fr[0];
-        const float wy = by ? fr[1] : 1.0f - fr[1];
-        const float wz = bz ? fr[2] : 1.0f - fr[2];
-        w[c] = (wx * wy) * wz;
+        idx[c] = grid_index_at(g.res[l], g.dense[l], g.log2_table, c0[0] + (c & 1u), c0[1] + ((c >> 1) & 1u),
+                               c0[2] + ((c >> 2) & 1u));
+        w[c] = corner_weight(fr, c);
     }
 }
 
-// Two features of level l (corner order, feature += w * value)
-__device__ __forceinline__ void level_features(const NrcGrid &g, const __half2 *table, uint32_t l, const float u[3],
-                                               float &f0, float &f1)
+// Word 3 of a raw 32-bit buffer descriptor on the gfx9 family.
+constexpr int kRawBufferWord3 = 0x00020000;
+
+// The encoding splits a record's 16 levels over 4 lanes: lane part (lane % 4)
+// of the record's group takes levels 4 part .. 4 part + 3. Their sizes are
+// fixed per lane, so a kernel loads them once (lane_levels) instead of per
+// record.
+struct LaneLevels {
+    uint32_t res[4];
+    uint32_t dense[4];
+};
+__device__ __forceinline__ LaneLevels lane_levels(const NrcGrid &g, uint32_t part)
 {
-    uint32_t idx[8];
-    float w[8];
-    level_corners(g, l, u, idx, w);
-    const __half2 *t = table + (size_t(l) << g.log2_table);
-    __half2 v[8];
-    for (int c = 0; c < 8; c++) v[c] = t[idx[c]];  // all gathers in flight together
-    f0 = 0.0f;
-    f1 = 0.0f;
-    for (int c = 0; c < 8; c++) {
-        const float2 fv = __half22float2(v[c]);
-        f0 = f0 + w[c] * fv.x;
-        f1 = f1 + w[c] * fv.y;
+    LaneLevels ll;
+    for (uint32_t j = 0; j < 4u; j++) {
+        ll.res[j] = g.res[4u * part + j];
+        ll.dense[j] = g.dense[4u * part + j];
+    }
+    return ll;
+}
+
+// The 8 grid features of levels 4 part .. 4 part + 3 at u, into f[2 j],
+// f[2 j + 1] for level 4 part + j: all 32 corner gathers are issued before
+// any is used, then each level's features are summed in corner order
+// (feature += w * value).
+__device__ __forceinline__ void lane_features(const NrcGrid &g, const LaneLevels &ll, const __half2 *table,
+                                              uint32_t part, const float u[3], float f[8])
+{
+    uint32_t idx[4][8];  // entry index in the whole table (levels x 2^log2_table < 2^32)
+    float fr[4][3];
+    for (uint32_t j = 0; j < 4u; j++) {
+        uint32_t c0[3];
+        cell_at(ll.res[j], u, c0, fr[j]);
+        const uint32_t level_base = (4u * part + j) << g.log2_table;
+        for (uint32_t c = 0; c < 8u; c++)
+            idx[j][c] = level_base + grid_index_at(ll.res[j], ll.dense[j], g.log2_table, c0[0] + (c & 1u),
+                                                   c0[1] + ((c >> 1) & 1u), c0[2] + ((c >> 2) & 1u));
+    }
+    // the gathers go through a buffer descriptor: a 32-bit offset register
+    // per load instead of a 64-bit address pair
+    const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<__half2 *>(table), 0, static_cast<int>((kLevels << g.log2_table) * 4u), kRawBufferWord3);
+    __half2 v[4][8];
+    for (uint32_t j = 0; j < 4u; j++)
+        for (uint32_t c = 0; c < 8u; c++)
+            v[j][c] = __builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(trs, idx[j][c] * 4u, 0, 0));
+    for (uint32_t j = 0; j < 4u; j++) {
+        float f0 = 0.0f, f1 = 0.0f;
+        for (uint32_t c = 0; c < 8u; c++) {
+            const float2 fv = __half22float2(v[j][c]);
+            const float w = corner_weight(fr[j], c);
+            f0 = f0 + w * fv.x;
+            f1 = f1 + w * fv.y;
+        }
+        f[2u * j] = f0;
+        f[2u * j + 1u] = f1;
     }
 }
 
@@ -235,12 +291,10 @@ __device__ __forceinline__ void layer_forward(const uint16_t *Wt, uint32_t n_til
         // acc[j] = Y[it*16 + r][nt*16 + 4q + j]
         if (out) {
             *out = acc;
-        } else {
-            uint16_t h[4];
-            for (int j = 0; j < 4; j++) h[j] = to_bf16(fmaxf(acc[j], 0.0f));
+        } else {  // relu leaves no NaN: pack_bf16_rne == to_bf16
             uint2 v;
-            v.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-            v.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+            v.x = pack_bf16_rne(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f));
+            v.y = pack_bf16_rne(fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
             *reinterpret_cast<uint2 *>(Y + (it * 16u + r) * kStride + nt * 16u + 4u * q) = v;
         }
     }
@@ -285,39 +339,55 @@ __device__ __forceinline__ uint32_t tile_record(uint32_t base, uint32_t s, uint3
     return wrap ? (base + s) % wrap : base + s;
 }
 
+// 8 consecutive bf16 values to a 16-byte aligned place
+__device__ __forceinline__ void store8_bf16(uint16_t *dst, const float v[8])
+{
+    uint32_t w[4];
+    for (int k = 0; k < 4; k++) w[k] = uint32_t(to_bf16(v[2 * k])) | (uint32_t(to_bf16(v[2 * k + 1])) << 16);
+    *reinterpret_cast<uint4 *>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// The network inputs of record rc (index i) into its bf16 row, lane part of
+// the record's 4-lane group: grid features 8 part .. 8 part + 7 (levels
+// 4 part .. 4 part + 3) and inputs 32 + 8 part .. 32 + 8 part + 7.
+// fp32_out (optional) receives them before rounding, at row i. zero: the row
+// holds no record and the lane clears its 16 entries.
+__device__ __forceinline__ void encode_record(const NrcGrid &g, const LaneLevels &ll, const __half2 *table,
+                                              const NrcRecord *recs, uint32_t i, bool zero, uint32_t part,
+                                              uint16_t *row, float *fp32_out)
+{
+    if (zero) {
+        uint4 *r = reinterpret_cast<uint4 *>(row + part * 16u);
+        r[0] = make_uint4(0u, 0u, 0u, 0u);
+        r[1] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const NrcRecord rc = recs[i];  // the whole record in one round trip, ahead of the gathers
+    float u[3];
+    unit_position(g, rc.pos, u);
+    float f[8], e[8];
+    lane_features(g, ll, table, part, u, f);
+    for (uint32_t k = 0; k < 8u; k++) e[k] = extra_input(rc, kGridFeatures + part * 8u + k);
+    store8_bf16(row + part * 8u, f);
+    store8_bf16(row + kGridFeatures + part * 8u, e);
+    if (fp32_out) {
+        for (uint32_t k = 0; k < 8u; k++) {
+            fp32_out[size_t(i) * kIn + part * 8u + k] = f[k];
+            fp32_out[size_t(i) * kIn + kGridFeatures + part * 8u + k] = e[k];
+        }
+    }
+}
+
 // Network inputs of a 64-sample tile into X (bf16 rows); rows without a
-// sample are zeros. 4 lanes per sample, 4 levels each; the first of them
-// also writes inputs 32..63. fp32_out (optional) receives the inputs before
-// rounding, at the record's index.
+// sample are zeros. 4 lanes per sample (encode_record).
 __device__ __forceinline__ void encode_tile(const NrcGrid &g, const __half2 *table, const NrcRecord *recs,
                                             uint32_t base, uint32_t valid, uint32_t wrap, uint16_t *X,
                                             float *fp32_out)
 {
     const uint32_t s = threadIdx.x / 4u, part = threadIdx.x % 4u;
-    uint16_t *row = X + s * kStride;
-    if (s >= valid) {
-        for (uint32_t k = part * 16u; k < part * 16u + 16u; k++) row[k] = 0u;
-        return;
-    }
-    const uint32_t i = tile_record(base, s, wrap);
-    const NrcRecord &rc = recs[i];
-    float u[3];
-    unit_position(g, rc.pos, u);
-    for (uint32_t l = part * 4u; l < part * 4u + 4u; l++) {
-        float f0, f1;
-        level_features(g, table, l, u, f0, f1);
-        row[2u * l] = to_bf16(f0);
-        row[2u * l + 1u] = to_bf16(f1);
-        if (fp32_out) {
-            fp32_out[size_t(i) * kIn + 2u * l] = f0;
-            fp32_out[size_t(i) * kIn + 2u * l + 1u] = f1;
-        }
-    }
-    for (uint32_t k = kGridFeatures + part * 8u; k < kGridFeatures + part * 8u + 8u; k++) {
-        const float v = extra_input(rc, k);
-        row[k] = to_bf16(v);
-        if (fp32_out) fp32_out[size_t(i) * kIn + k] = v;
-    }
+    const LaneLevels ll = lane_levels(g, part);
+    encode_record(g, ll, table, recs, s < valid ? tile_record(base, s, wrap) : 0u, s >= valid, part,
+                  X + s * kStride, fp32_out);
 }
 
 __device__ __forceinline__ void load_weights(const uint16_t *__restrict__ wt, uint16_t *s_wt)
@@ -393,12 +463,10 @@ __device__ __forceinline__ void rows_forward(const uint16_t *Wt, uint32_t n_tile
         *out = acc[0];
         return;
     }
-    for (uint32_t nt = 0; nt < n_tiles; nt++) {
-        uint16_t h[4];
-        for (int j = 0; j < 4; j++) h[j] = to_bf16(fmaxf(acc[nt][j], 0.0f));
+    for (uint32_t nt = 0; nt < n_tiles; nt++) {  // relu leaves no NaN: pack_bf16_rne == to_bf16
         uint2 v;
-        v.x = uint32_t(h[0]) | (uint32_t(h[1]) << 16);
-        v.y = uint32_t(h[2]) | (uint32_t(h[3]) << 16);
+        v.x = pack_bf16_rne(fmaxf(acc[nt][0], 0.0f), fmaxf(acc[nt][1], 0.0f));
+        v.y = pack_bf16_rne(fmaxf(acc[nt][2], 0.0f), fmaxf(acc[nt][3], 0.0f));
         *reinterpret_cast<uint2 *>(X + r * kStride + nt * 16u + 4u * q) = v;
     }
 }
@@ -433,35 +501,12 @@ __global__ __launch_bounds__(64u * kInferWaves) void nrc_infer_kernel(NrcGrid g,
     }
     __syncthreads();
     uint16_t *X = s_x[wave];
+    const LaneLevels ll = lane_levels(g, lane % 4u);
     for (uint32_t tile = blockIdx.x * kInferWaves + wave; tile * 16u < n; tile += gridDim.x * kInferWaves) {
         const uint32_t first = tile * 16u;
-        {  // encoding: lane group s = lane / 4 holds record first + s, 4 levels per lane
-            const uint32_t sr = lane / 4u, part = lane % 4u;
-            uint16_t *row = X + sr * kStride;
-            if (first + sr >= n) {
-                for (uint32_t k = part * 16u; k < part * 16u + 16u; k++) row[k] = 0u;
-            } else {
-                const uint32_t i = first + sr;
-                const NrcRecord &rc = recs[i];
-                float u[3];
-                unit_position(g, rc.pos, u);
-                for (uint32_t l = part * 4u; l < part * 4u + 4u; l++) {
-                    float f0, f1;
-                    level_features(g, table, l, u, f0, f1);
-                    row[2u * l] = to_bf16(f0);
-                    row[2u * l + 1u] = to_bf16(f1);
-                    if (inputs) {
-                        inputs[size_t(i) * kIn + 2u * l] = f0;
-                        inputs[size_t(i) * kIn + 2u * l + 1u] = f1;
-                    }
-                }
-                for (uint32_t k = kGridFeatures + part * 8u; k < kGridFeatures + part * 8u + 8u; k++) {
-                    const float v = extra_input(rc, k);
-                    row[k] = to_bf16(v);
-                    if (inputs) inputs[size_t(i) * kIn + k] = v;
-                }
-            }
-        }
+        // encoding: lane group sr = lane / 4 holds record first + sr
+        const uint32_t sr = lane / 4u;
+        encode_record(g, ll, table, recs, first + sr, first + sr >= n, lane % 4u, X + sr * kStride, inputs);
         // The rows are this wave's alone: its LDS accesses complete in the
         // order it issues them, so only the compiler must keep that order (a
         // wavefront-scope fence emits no instruction).

@@ -573,6 +573,7 @@ __global__ void math_kernel(int op, const float *a, const float *b, float *out, 
     case 1: { float s, c; sin_cos(a[i], s, c); r = c; break; }
     case 2: r = pow_f(a[i], b[i]); break;
     case 3: r = acos_f(a[i]); break;
+    case 5: r = __uint_as_float(pack_bf16_rne(a[i], 0.0f) << 16); break;
     default: r = atan2_f(a[i], b[i]); break;
     }
     out[i] = r;
@@ -1522,7 +1523,7 @@ orn_status orn_trace_rays_packet(orn_device_scene *s, float t_min, const float *
 orn_status orn_eval_math(orn_ctx *ctx, int op, const float *a, const float *b, float *out, size_t n)
 {
     if (!ctx) return ORN_ERROR_INVALID_HANDLE;
-    if (op < 0 || op > 4 || !a || !out || ((op == 2 || op == 4) && !b)) return ORN_ERROR_INVALID_VALUE;
+    if (op < 0 || op > 5 || !a || !out || ((op == 2 || op == 4) && !b)) return ORN_ERROR_INVALID_VALUE;
     if (n == 0) return ORN_OK;
     if (n > 0xffffffffull) return ORN_ERROR_INVALID_VALUE;
     ORN_TRY(hipSetDevice(ctx->device));

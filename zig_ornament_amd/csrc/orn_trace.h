@@ -857,4 +857,16 @@ __device__ __forceinline__ bool scatter(const DevScene &sc, const Surface &srf, 
     }
 }
 
+// Two floats to packed bf16 (first in the low half) with gfx950's
+// v_cvt_pk_bf16_f32, which rounds to nearest even. For every non-NaN input
+// that is the software rounding (u + 0x7fff + ((u >> 16) & 1)) >> 16 of the
+// radiance cache's numerics; a NaN input may come out with another NaN
+// payload. orn_eval_math op 5 evaluates it for the tests.
+__device__ __forceinline__ uint32_t pack_bf16_rne(float a, float b)
+{
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t v = {static_cast<__bf16>(a), static_cast<__bf16>(b)};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
 }  // namespace orn
