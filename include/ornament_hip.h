@@ -249,7 +249,12 @@ orn_status orn_target_wait(orn_target *target);
  *   before its wave looks for leaf and shading rounds again; fewer checks
  *   cost fewer instructions, while lanes that reach a leaf or finish their
  *   ray early wait longer (0 takes the default, 4).
- * reserved: must be 0. */
+ * workgroup_pool (0 or 1): 1 pools the rays at internal nodes of a
+ *   workgroup's 4 waves into as few waves as hold them before each run of
+ *   node steps (state through LDS, two barriers per run), and shades with all
+ *   4 waves at once, triggered when regroup_lanes x 4 of the workgroup's lanes
+ *   wait. It applies to scenes of spheres only with the BVH2 and is ignored
+ *   elsewhere (default 0). */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
@@ -258,7 +263,7 @@ typedef struct orn_tuning {
     uint32_t samples_per_job;
     uint32_t split_percent;
     uint32_t node_steps;
-    uint32_t reserved;
+    uint32_t workgroup_pool;
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);

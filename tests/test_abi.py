@@ -165,4 +165,5 @@ def test_shipped_tuning_cache_entries_are_valid():
         assert e["workgroups_per_cu"] <= 64 and e["bvh_width"] in (2, 4), k
         assert e["samples_per_job"] <= 0xFFFF and e["split_percent"] <= 100, k
         assert 0 <= e.get("node_steps", 0) <= 16, k
+        assert e.get("workgroup_pool", 0) in (0, 1), k
         assert e["ms"] > 0, k

@@ -136,7 +136,7 @@ def test_device_errors(device):
 
 def test_tuning_validation_and_defaults(device):
     """orn_target_set_tuning rejects out-of-range fields (node_steps above
-    16, a nonzero reserved word) and stores node_steps 0 as its default."""
+    16, workgroup_pool above 1) and stores node_steps 0 as its default."""
     import ctypes as C
     from zig_ornament_amd import _native as N
     spec = W.cornell(width=16, height=16, spp=1)
@@ -154,7 +154,7 @@ def test_tuning_validation_and_defaults(device):
         assert e.value.status == 1, bad  # ORN_ERROR_INVALID_VALUE
     raw = N.Tuning()
     assert N.lib().orn_target_get_tuning(t._h, C.byref(raw)) == 0
-    raw.reserved = 1
+    raw.workgroup_pool = 2
     assert N.lib().orn_target_set_tuning(t._h, C.byref(raw)) == 1  # ORN_ERROR_INVALID_VALUE
     assert t.tuning()["node_steps"] == 16
 

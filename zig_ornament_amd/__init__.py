@@ -432,7 +432,7 @@ class DeviceScene:
 
 
 def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16,
-                   samples_per_job=16, split_percent=0, node_steps=4):
+                   samples_per_job=16, split_percent=0, node_steps=4, workgroup_pool=0):
     t = N.Tuning()
     t.regroup_lanes = int(regroup_lanes)
     t.workgroups_per_cu = int(workgroups_per_cu)
@@ -441,6 +441,7 @@ def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regr
     t.samples_per_job = int(samples_per_job)
     t.split_percent = int(split_percent)
     t.node_steps = int(node_steps)
+    t.workgroup_pool = int(workgroup_pool)
     return t
 
 
@@ -448,7 +449,7 @@ def _tuning_dict(t):
     return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
             "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes,
             "samples_per_job": t.samples_per_job, "split_percent": t.split_percent,
-            "node_steps": t.node_steps}
+            "node_steps": t.node_steps, "workgroup_pool": t.workgroup_pool}
 
 
 class Target:
@@ -474,9 +475,9 @@ class Target:
         check(lib().orn_target_wait(self._h), "orn_target_wait")
 
     def set_tuning(self, regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16,
-                   samples_per_job=16, split_percent=0, node_steps=4):
+                   samples_per_job=16, split_percent=0, node_steps=4, workgroup_pool=0):
         t = _tuning_struct(regroup_lanes, workgroups_per_cu, bvh_width, leaf_regroup_lanes,
-                           samples_per_job, split_percent, node_steps)
+                           samples_per_job, split_percent, node_steps, workgroup_pool)
         check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
 
     def tuning(self):

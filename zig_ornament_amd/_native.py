@@ -65,7 +65,7 @@ class SceneDesc(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("regroup_lanes", u32), ("workgroups_per_cu", u32), ("bvh_width", u32),
                 ("leaf_regroup_lanes", u32), ("samples_per_job", u32), ("split_percent", u32), ("node_steps", u32),
-                ("reserved", u32)]
+                ("workgroup_pool", u32)]
 
 
 class RenderSettings(C.Structure):

@@ -163,7 +163,40 @@ __device__ __forceinline__ void store_coherent(__amdgpu_buffer_rsrc_t r, uint32_
 // BATCH: the tiles come from a work list over many images (BatchTiles), and
 // each lane reads its image's parameters from the BatchItem table; the
 // pixel buffers are pools indexed by px_base + pixel.
-template <uint32_t F, uint32_t W, bool BATCH>
+//
+// POOL (tuning workgroup_pool; spheres-only scenes, BVH2, one image): before
+// each run of node steps the rays at internal nodes of the workgroup's 4
+// waves are pooled: every such lane writes its traversal state to its slot in
+// LDS, and after a barrier the waves take the slots in order, 64 per wave, and
+// run the node steps on the owners' stack columns; after a second barrier the
+// owners reload node and stack top. Leaf and shading rounds stay with the
+// owners, and a shading round is taken by all waves together (workgroup-wide
+// trigger). Each ray's operations are unchanged, so are the results. It fills
+// the node steps' lanes (lane occupancy 0.39 -> 0.50 on random spheres) but
+// costs as many instructions as it saves, and the barriers cost issue
+// cycles: it is an option, not the default.
+// Position of the k-th (from 0) set bit of m; m has more than k set bits.
+__device__ __forceinline__ uint32_t kth_set_bit(uint64_t m, uint32_t k)
+{
+    uint32_t pos = 0, c;
+    c = __popc(static_cast<uint32_t>(m));
+    if (k >= c) { k -= c; m >>= 32; pos += 32; }
+    c = __popc(static_cast<uint32_t>(m) & 0xffffu);
+    if (k >= c) { k -= c; m >>= 16; pos += 16; }
+    c = __popc(static_cast<uint32_t>(m) & 0xffu);
+    if (k >= c) { k -= c; m >>= 8; pos += 8; }
+    c = __popc(static_cast<uint32_t>(m) & 0xfu);
+    if (k >= c) { k -= c; m >>= 4; pos += 4; }
+    c = __popc(static_cast<uint32_t>(m) & 0x3u);
+    if (k >= c) { k -= c; m >>= 2; pos += 2; }
+    c = static_cast<uint32_t>(m) & 1u;
+    if (k >= c) pos += 1;
+    return pos;
+}
+// node reference of a pooled traversal that has finished
+constexpr uint32_t kDoneRef = 0xffffffffu;
+
+template <uint32_t F, uint32_t W, bool BATCH, bool POOL = false>
 __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParams kp,
                                                         const BatchItem *__restrict__ items,
                                                         const BatchTiles *__restrict__ work,
@@ -350,15 +383,96 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     const uint32_t regroup = kp.regroup;
     const uint32_t leaf_regroup = kp.leaf_regroup;
     const uint32_t node_steps = kp.node_steps;
+    bool shade_all = false;  // POOL: this iteration's workgroup-wide shading decision
     for (;;) {
-        // up to `node_steps` internal nodes per lane every iteration; leaves
-        // (sphere, triangle, mesh entry: the long code) in rounds of at
-        // least `leaf_regroup` lanes. An unflagged tr.addr implies
-        // mode == kTraverse: lanes that stop traversing take kLeafRef.
-        for (uint32_t k = 0; k < node_steps; k++) {
-            if (!(tr.addr & kLeafRef) && !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr)) {
-                mode = kWaiting;
-                tr.addr = kLeafRef;
+        if constexpr (POOL) {
+            static_assert(!(F & kHasMeshes) && W == 2u && !BATCH, "pooled traversal: spheres, BVH2, one image");
+            __shared__ float p_f[7][kBlock];  // inv xyz, ox xyz, t_max of each slot
+            __shared__ uint32_t p_addr[kBlock], p_top[kBlock];
+            __shared__ uint64_t p_mask[kBlock / 64u];
+            __shared__ uint32_t p_alive[kBlock / 64u], p_wait[kBlock / 64u], p_trav[kBlock / 64u];
+            const uint32_t wave = threadIdx.x / 64u;
+            const bool inner = !(tr.addr & kLeafRef);
+            const uint64_t im = lanes_where(inner);
+            const bool wave_alive = lanes_where(mode != kDone) != 0u;
+            const uint32_t nw = static_cast<uint32_t>(__popcll(lanes_where(mode == kWaiting)));
+            const uint32_t nt = static_cast<uint32_t>(__popcll(lanes_where(mode == kTraverse)));
+            if (inner) {
+                p_f[0][threadIdx.x] = tr.inv.x;
+                p_f[1][threadIdx.x] = tr.inv.y;
+                p_f[2][threadIdx.x] = tr.inv.z;
+                p_f[3][threadIdx.x] = tr.ox.x;
+                p_f[4][threadIdx.x] = tr.ox.y;
+                p_f[5][threadIdx.x] = tr.ox.z;
+                p_f[6][threadIdx.x] = tr.t_max;
+                p_addr[threadIdx.x] = tr.addr;
+                p_top[threadIdx.x] = tr.top;
+            }
+            if (lane == 0u) {
+                p_mask[wave] = im;
+                p_alive[wave] = wave_alive ? 1u : 0u;
+                p_wait[wave] = nw;
+                p_trav[wave] = nt;
+            }
+            __syncthreads();
+            // every wave leaves together once none has work
+            if ((p_alive[0] | p_alive[1] | p_alive[2] | p_alive[3]) == 0u) break;
+            const uint32_t ws = p_wait[0] + p_wait[1] + p_wait[2] + p_wait[3];
+            const uint32_t ts = p_trav[0] + p_trav[1] + p_trav[2] + p_trav[3];
+            shade_all = ws >= 4u * regroup || (ts == 0u ? ws > 0u : 2u * ts < ws);
+            const uint32_t n0 = static_cast<uint32_t>(__popcll(p_mask[0]));
+            const uint32_t n1 = static_cast<uint32_t>(__popcll(p_mask[1]));
+            const uint32_t n2 = static_cast<uint32_t>(__popcll(p_mask[2]));
+            const uint32_t n3 = static_cast<uint32_t>(__popcll(p_mask[3]));
+            const uint32_t total = n0 + n1 + n2 + n3;
+            if (wave * 64u < total) {  // wave-uniform
+                const uint32_t e = wave * 64u + lane;
+                const bool mine = e < total;
+                uint32_t slot = 0;
+                Traversal wt;
+                wt.addr = kDoneRef;
+                wt.top = 0;
+                wt.in_tlas = true;
+                wt.hit = false;
+                if (mine) {
+                    const uint32_t p1 = n0, p2 = n0 + n1, p3 = n0 + n1 + n2;
+                    const uint32_t ow = (e >= p1 ? 1u : 0u) + (e >= p2 ? 1u : 0u) + (e >= p3 ? 1u : 0u);
+                    const uint32_t before = ow == 0u ? 0u : (ow == 1u ? p1 : (ow == 2u ? p2 : p3));
+                    slot = ow * 64u + kth_set_bit(p_mask[ow], e - before);
+                    wt.inv = mk(p_f[0][slot], p_f[1][slot], p_f[2][slot]);
+                    wt.ox = mk(p_f[3][slot], p_f[4][slot], p_f[5][slot]);
+                    wt.t_max = p_f[6][slot];
+                    wt.addr = p_addr[slot];
+                    wt.top = p_top[slot];
+                }
+                for (uint32_t k = 0; k < node_steps; k++) {
+                    if (!(wt.addr & kLeafRef) && !trav_internal<F, W>(sc, t_min(), ray, lds_stack + slot, kBlock, wt))
+                        wt.addr = kDoneRef;
+                }
+                if (mine) {
+                    p_addr[slot] = wt.addr;
+                    p_top[slot] = wt.top;
+                }
+            }
+            __syncthreads();
+            if (inner) {
+                tr.addr = p_addr[threadIdx.x];
+                tr.top = p_top[threadIdx.x];
+                if (tr.addr == kDoneRef) {
+                    mode = kWaiting;
+                    tr.addr = kLeafRef;
+                }
+            }
+        } else {
+            // up to `node_steps` internal nodes per lane every iteration;
+            // leaves (sphere, triangle, mesh entry: the long code) in rounds
+            // of at least `leaf_regroup` lanes. An unflagged tr.addr implies
+            // mode == kTraverse: lanes that stop traversing take kLeafRef.
+            for (uint32_t k = 0; k < node_steps; k++) {
+                if (!(tr.addr & kLeafRef) && !trav_internal<F, W>(sc, t_min(), ray, stack, kBlock, tr)) {
+                    mode = kWaiting;
+                    tr.addr = kLeafRef;
+                }
             }
         }
         // masks of single comparisons, combined on the scalar unit
@@ -374,13 +488,20 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         }
         const uint64_t waiting = lanes_where(mode == kWaiting);
         const uint64_t traversing = lanes_where(mode == kTraverse);
-        if (waiting == 0u && traversing == 0u) break;
+        if (waiting == 0u && traversing == 0u) {
+            if constexpr (POOL) continue;  // the workgroup leaves together, at the loop's top
+            else break;
+        }
         // a shading round once `regroup` lanes wait, or once fewer than half
         // as many lanes still traverse as wait (the wave would otherwise step
         // a few lanes while most of it idles)
         const uint32_t n_waiting = static_cast<uint32_t>(__popcll(waiting));
-        if (n_waiting < regroup && traversing != 0u && 2u * static_cast<uint32_t>(__popcll(traversing)) >= n_waiting)
+        if constexpr (POOL) {
+            if (!shade_all) continue;
+        } else if (n_waiting < regroup && traversing != 0u &&
+                   2u * static_cast<uint32_t>(__popcll(traversing)) >= n_waiting) {
             continue;
+        }
         // shading round
         flush_releases();
         if (mode == kWaiting && !need_pixel) {
@@ -834,7 +955,7 @@ bool tuning_valid(const orn_tuning *t)
     return t && t->regroup_lanes >= 1 && t->regroup_lanes <= 64 && t->workgroups_per_cu <= 64 &&
            (t->bvh_width == 2u || t->bvh_width == 4u) && t->leaf_regroup_lanes >= 1 &&
            t->leaf_regroup_lanes <= 64 && t->samples_per_job <= 0xffffu && t->split_percent <= 100u &&
-           t->node_steps <= 16u && t->reserved == 0u;
+           t->node_steps <= 16u && t->workgroup_pool <= 1u;
 }
 
 // node_steps 0 stands for the default
@@ -877,18 +998,19 @@ struct LaunchArgs {
     uint32_t *stage;
     uint32_t *queues;
     uint32_t total_tiles;
+    bool pool;  // tuning workgroup_pool (honoured for spheres-only scenes, BVH2, one image)
 };
 
 // One instantiation per combination of scene features.
 // Persistent grid: workgroups_per_cu per compute unit (0: as many as can be
 // resident at once), a multiple of the XCD count, at most one per tile.
-template <uint32_t F, uint32_t W, bool BATCH>
-hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
+template <uint32_t F, uint32_t W, bool BATCH, bool POOL>
+hipError_t launch_render_k(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
                            const LaunchArgs &a)
 {
     if (per_cu == 0) {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F, W, BATCH>, kBlock, lds) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, render_kernel<F, W, BATCH, POOL>, kBlock, lds) !=
                 hipSuccess ||
             blocks < 1) {
             (void)hipGetLastError();  // the launch below reports its own errors
@@ -899,9 +1021,18 @@ hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_
     uint32_t grid = cus * per_cu;
     grid = std::min(grid, blocks_for(a.total_tiles, kXcds) * kXcds);
     grid = std::max(kXcds, grid / kXcds * kXcds);
-    render_kernel<F, W, BATCH>
+    render_kernel<F, W, BATCH, POOL>
         <<<grid, kBlock, lds, st>>>(ds, a.kp, a.items, a.work, a.acc, a.rng, a.fb, a.stage, a.queues);
     return hipGetLastError();
+}
+
+template <uint32_t F, uint32_t W, bool BATCH>
+hipError_t launch_render_f(uint32_t per_cu, uint32_t cus, size_t lds, hipStream_t st, const DevScene &ds,
+                           const LaunchArgs &a)
+{
+    if constexpr (!(F & kHasMeshes) && W == 2u && !BATCH)
+        if (a.pool) return launch_render_k<F, W, BATCH, true>(per_cu, cus, lds, st, ds, a);
+    return launch_render_k<F, W, BATCH, false>(per_cu, cus, lds, st, ds, a);
 }
 
 template <bool BATCH>
@@ -1330,7 +1461,7 @@ hipError_t enqueue_render(orn_device_scene *s, const orn_constant_params *params
     kp.epoch = epoch;
     kp.split_pct = tuning.split_percent;
     kp.pixels = params->width * params->height;
-    LaunchArgs a{kp, nullptr, nullptr, acc, rng, fb, stage, queues, kp.tiles};
+    LaunchArgs a{kp, nullptr, nullptr, acc, rng, fb, stage, queues, kp.tiles, tuning.workgroup_pool != 0u};
     reset_queues_kernel<<<1, 64, 0, st>>>(queues);
     return launch_render<false>(s->features, width, tuning.workgroups_per_cu,
                                 static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a);
@@ -2176,6 +2307,7 @@ bool cached_tuning(const char *path, const std::string &key, orn_tuning *t, floa
             else if (f.first == "samples_per_job") r.samples_per_job = v;
             else if (f.first == "split_percent") r.split_percent = v;
             else if (f.first == "node_steps") r.node_steps = v;
+            else if (f.first == "workgroup_pool") r.workgroup_pool = v;
             else if (f.first == "ms") *ms = static_cast<float>(f.second);
         }
         if (!(ok[0] && ok[1] && ok[2] && ok[3])) return false;
@@ -2198,6 +2330,7 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
         {"samples_per_job", t.samples_per_job},
         {"split_percent", t.split_percent},
         {"node_steps", t.node_steps},
+        {"workgroup_pool", t.workgroup_pool},
         {"ms", ms}};
     bool found = false;
     for (auto &e : c)
