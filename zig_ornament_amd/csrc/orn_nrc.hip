@@ -1244,9 +1244,8 @@ __global__ __launch_bounds__(kBlock) void nrc_compact_kernel(const NrcRecord *__
 
 // render() bookkeeping of one sample per pixel (pathtracer.wgsl:64-79):
 // current_iteration + 1, then overwrite or add.
-// One sample per pixel into the accumulation (pathtracer.wgsl:64-79); with
-// `post` (the render's last iteration) also the framebuffer
-// (pathtracer.wgsl:81-93) from the new accumulation.
+// Clears one of the counters (queries, training records) on the stream that
+// appends to it.
 __global__ void nrc_reset_count_kernel(uint32_t *c)
 {
     if (threadIdx.x == 0) *c = 0u;
