@@ -47,7 +47,8 @@ Also reported:
                icosphere, two 1024^2 textures, 1920x1080, depth 8): the cache
                trains online along a seeded 120-frame camera path at 1 spp per
                frame; the last frame's 1-spp image is compared (RMSE and
-               relMSE on linear radiance) with a 4,096-spp plain render of that
+               relMSE on linear radiance, and RMSE of the post-processed
+               framebuffers) with a 4,096-spp plain render of that
                view, next to plain renders of the same view at increasing spp:
                the plain RMSE in the cache's frame time, and the plain time
                needed to reach the cache's RMSE. Two training schedules:
@@ -222,6 +223,7 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
         done += k
     acc = ref.accumulation()
     reference = acc / acc[:, 3:4]
+    reference_fb = ref.framebuffer()  # post-processed (pathtracer.wgsl:81-93)
 
     runs = {}
     for name, cfg, steps in NRC_RUNS:
@@ -233,6 +235,7 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
             frame_ms.append(ms)
         img = t.accumulation()
         runs[name] = {"rmse": rmse(img, reference), "relmse": relmse(img, reference),
+                      "rmse_framebuffer": rmse(t.framebuffer(), reference_fb),
                       "ms_per_frame": float(np.median(frame_ms[frames // 2:])), "spp": 1,
                       "query_bounce": 0, "log2_table": cfg["log2_table"],
                       "train_paths_per_frame": cfg["train_paths"], "adam_steps_per_frame": steps,
@@ -252,7 +255,7 @@ def equal_time_rmse(dev, frames=120, ref_spp=4096):
             ms.append(pt.last_render_ms())
         img = pt.accumulation() / spp
         plain.append({"spp": spp, "ms": float(np.median(ms)), "rmse": rmse(img, reference),
-                      "relmse": relmse(img, reference)})
+                      "relmse": relmse(img, reference), "rmse_framebuffer": rmse(pt.framebuffer(), reference_fb)})
         if spp >= 8 and plain[-1]["rmse"] <= target and plain[-1]["ms"] > slowest:
             break
     out = {"scene": "instanced_icospheres_1920x1080_depth8", "frames": frames, "reference_spp": ref_spp}
