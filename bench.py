@@ -12,7 +12,11 @@ way (the profiled HBM traffic is reported for C2 only).
 Launch tuning: the library's autotuner picks the render kernel's launch
 parameters for the scene class; its JSON cache ships with the package
 (zig_ornament_amd/tuning_cache.json), so a known class is applied without
-trial renders. Tuning never changes the image.
+trial renders. Tuning never changes the image. C2's tuning queues each XCD's
+pixels by what they cost in the target's previous render (cost_order: most
+expensive first, so the lanes run out of work nearly together); the warm-up
+renders record those costs, and `first_render_kernel_ms` reports the
+target's first render, which has none and queues in tile order.
 
 Also reported:
   roofline     the bound that limits the render kernel: vector-instruction
@@ -408,8 +412,11 @@ def main():
         ms = dev.calibration_copy(nbytes)
         calib = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms,
                  "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
-    for _ in range(args.warmup):
+    first_ms = None
+    for i in range(args.warmup):
         tgt.render(params, spec.spp)
+        if i == 0:
+            first_ms = tgt.last_render_ms()  # no previous costs: tile-order queues
     tgt.wait()
     kernel_ms = []
     t0 = time.perf_counter()
@@ -443,6 +450,7 @@ def main():
                    "compute_units": cus, "bvh_build_s": build_s, "launch_tuning": tuning,
                    "autotuner_ms": tuning_ms},
         "kernel_ms_avg": avg_kernel_s * 1e3,
+        "first_render_kernel_ms": first_ms,
     }
     if calib:
         out["calibration_copy"] = calib

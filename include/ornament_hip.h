@@ -254,7 +254,16 @@ orn_status orn_target_wait(orn_target *target);
  *   node steps (state through LDS, two barriers per run), and shades with all
  *   4 waves at once, triggered when regroup_lanes x 4 of the workgroup's lanes
  *   wait. It applies to scenes of spheres only with the BVH2 and is ignored
- *   elsewhere (default 0). */
+ *   elsewhere (default 0).
+ * cost_order (0 or 1): 1 queues each XCD's pixels most expensive first, by
+ *   what they cost in the target's previous render with cost_order 1 (GPU
+ *   time of their jobs, in quarter-octave classes; tile order within a
+ *   class), so that the cheapest pixels are the last ones drawn and the end
+ *   of the render, where lanes run out of work, is short (longest processing
+ *   time first). The first such render of a target queues in tile order and
+ *   records the costs; every such render records them for the next. It
+ *   applies to orn_render and orn_render_async and is ignored by batches and
+ *   orn_render_noalloc (default 0). */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
@@ -264,6 +273,7 @@ typedef struct orn_tuning {
     uint32_t split_percent;
     uint32_t node_steps;
     uint32_t workgroup_pool;
+    uint32_t cost_order;
 } orn_tuning;
 orn_status orn_target_set_tuning(orn_target *target, const orn_tuning *tuning);
 orn_status orn_target_get_tuning(orn_target *target, orn_tuning *tuning);
@@ -299,6 +309,12 @@ orn_status orn_target_last_render_ms(orn_target *target, float *ms);
 orn_status orn_get_framebuffer(orn_target *target, float *dst, size_t pixels);
 orn_status orn_get_accumulation(orn_target *target, float *dst, size_t pixels);
 orn_status orn_get_rng_state(orn_target *target, uint32_t *dst, size_t pixels);
+/* Per-pixel cost recorded by the target's latest render with tuning
+ * cost_order 1: the GPU time of the pixel's jobs, in ticks of the 100 MHz
+ * device clock (width * height entries, row-major like the accumulation).
+ * No counterpart in the reference; it is what orders the next render's
+ * queues. ORN_ERROR_INVALID_VALUE if no render has recorded costs. */
+orn_status orn_get_pixel_costs(orn_target *target, uint32_t *dst, size_t pixels);
 /* Restore a saved accumulation + RNG state (checkpoint / resume of a
  * progressive render; either pointer may be NULL to keep the current one). */
 orn_status orn_set_state(orn_target *target, const float *accumulation, const uint32_t *rng,
@@ -356,8 +372,9 @@ orn_status orn_batch_set_state(orn_batch *batch, size_t item, const float *accum
 /* Times renders of `iterations` samples per pixel (a few suffice) under
  * candidate tunings (coordinate descent over regroup_lanes,
  * leaf_regroup_lanes, node_steps, workgroups_per_cu, split_percent,
- * samples_per_job and, with ORN_TUNE_ALLOW_BVH4, bvh_width; then the
- * neighbours of the best regroup_lanes, leaf_regroup_lanes and node_steps),
+ * samples_per_job, cost_order (targets only) and, with ORN_TUNE_ALLOW_BVH4,
+ * bvh_width; then the neighbours of the best regroup_lanes,
+ * leaf_regroup_lanes and node_steps),
  * sets the fastest on the target and returns it with its time.
  * The target's accumulation, RNG state and framebuffer are left as they
  * were. With a cache_path (a JSON file, created if absent), a tuning stored

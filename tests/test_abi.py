@@ -166,4 +166,5 @@ def test_shipped_tuning_cache_entries_are_valid():
         assert e["samples_per_job"] <= 0xFFFF and e["split_percent"] <= 100, k
         assert 0 <= e.get("node_steps", 0) <= 16, k
         assert e.get("workgroup_pool", 0) in (0, 1), k
+        assert e.get("cost_order", 0) in (0, 1), k
         assert e["ms"] > 0, k

@@ -15,10 +15,12 @@ from zig_ornament_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def check_subset(prod, spec, acc_g, rng_g, pixels):
+def check_subset(prod, spec, acc_g, rng_g, pixels, rng0=None):
+    """The oracle renders `pixels` from their seeded RNG state, or from
+    rng0[pixels] (the state an earlier render left)."""
     n = len(pixels)
     acc = np.zeros((n, 4), np.float32)
-    rng = pixels.astype(np.uint32).copy()
+    rng = pixels.astype(np.uint32).copy() if rng0 is None else rng0[pixels].copy()
     p = O.ConstantParams.from_buffer_copy(bytes(zo_params(spec)))
     O.OracleScene(prod).render_pixels(p, spec.spp, pixels, acc, rng)
     assert same_bits(acc, acc_g[pixels])
@@ -38,10 +40,11 @@ def cached_entry(tuning, ms):
 @pytest.mark.parametrize("name", ["C2", "C3"])
 def test_full_size_render(device, name):
     """The launch bench.py times (the shipped tuning-cache entry of the
-    scene class: split pixels handed between lanes, several nodes per round)
-    at full size; the spp/2 + spp/2 render uses a different schedule without
-    pixel jobs and must give the same bits, and a seeded subset must equal
-    the oracle."""
+    scene class: several nodes per round; for C2, queues ordered by the
+    previous render's pixel costs) at full size; the spp/2 + spp/2 render
+    uses a different schedule and must give the same bits, a seeded subset
+    must equal the oracle, and so must a subset of the second, cost-ordered
+    render."""
     spec = W.random_spheres() if name == "C2" else W.instanced_icospheres()
     print(f"[full] {name} {spec.width}x{spec.height} {spec.spp} spp depth {spec.depth}", flush=True)
     prod = zo.Bvh(W.build_scene(spec))
@@ -51,8 +54,8 @@ def test_full_size_render(device, name):
     entry = cached_entry(tuning, ms)
     print(f"[full] {name} timed launch tuning {entry}", flush=True)
     assert a.tuning() == tuning
-    if name == "C2":  # the headline launch hands split pixels from job to job
-        assert tuning["split_percent"] > 0 and 0 < tuning["samples_per_job"] < spec.spp
+    if name == "C2":  # the headline launch queues pixels by their previous cost
+        assert tuning["cost_order"] == 1
     a.render(zo_params(spec), spec.spp)
     acc_a, rng_a = a.accumulation(), a.rng_state()
     print(f"[full] {name} render {a.last_render_ms():.1f} ms", flush=True)
@@ -76,6 +79,16 @@ def test_full_size_render(device, name):
     pixels = np.random.default_rng(0).choice(spec.width * spec.height, 1500, replace=False)
     pixels = np.unique(np.concatenate([pixels, np.nonzero(bad)[0]])).astype(np.uint32)
     check_subset(prod, spec, acc_a, rng_a, pixels)
+    if tuning["cost_order"]:
+        # bench.py times renders that follow another one: their queues are
+        # ordered by the previous render's pixel costs
+        assert np.all(a.pixel_costs() > 0)
+        a.render(zo_params(spec), spec.spp)
+        print(f"[full] {name} cost-ordered render {a.last_render_ms():.1f} ms", flush=True)
+        acc_2, rng_2 = a.accumulation(), a.rng_state()
+        assert np.all(acc_2[:, 3] == spec.spp)
+        sub = np.random.default_rng(1).choice(spec.width * spec.height, 500, replace=False).astype(np.uint32)
+        check_subset(prod, spec, acc_2, rng_2, np.unique(sub), rng0=rng_a)
 
 
 def test_cornell_4096_spp_convergence(device):

@@ -162,6 +162,39 @@ def test_image_matches_oracle(device, name, tuning):
     assert same_bits(fb_g, fb_o)
 
 
+@pytest.mark.parametrize("tuning", [dict(cost_order=1),
+                                    dict(cost_order=1, samples_per_job=3, split_percent=100, regroup_lanes=8),
+                                    dict(cost_order=1, samples_per_job=2, split_percent=60, node_steps=6)])
+@pytest.mark.parametrize("name", ["C1", "C2s", "C3s"])
+def test_cost_ordered_renders_match_oracle(device, name, tuning):
+    """With cost_order 1 the first render records each pixel's cost and
+    every later one queues the pixels by the previous render's costs (a
+    permutation of each band's queue, built on the GPU). Three progressive
+    renders must each equal the oracle bit for bit: a pixel queued twice or
+    never would leave a wrong accumulation or RNG state."""
+    spec = small_specs()[name]
+    prod, _ = both_arrays(spec)
+    n = spec.width * spec.height
+    ds = zo.DeviceScene(device, prod)
+    tgt = zo.Target(ds, spec.width, spec.height)
+    tgt.set_tuning(**tuning)
+    with pytest.raises(zo.OrnamentError):
+        tgt.pixel_costs()  # nothing recorded yet
+    acc_o = np.zeros((n, 4), np.float32)
+    rng_o = O.seed_rng(n)
+    oracle = O.OracleScene(prod)
+    for k in range(3):
+        print(f"[ordered] {name} render {k}, {tuning}", flush=True)
+        params = zo_params(spec, current_iteration=float(k * spec.spp))
+        tgt.render(params, spec.spp)
+        costs = tgt.pixel_costs()
+        assert np.all(costs > 0)
+        fb_o, _ = oracle.render(to_oracle_params(params), spec.spp, acc_o, rng_o)
+        assert same_bits(tgt.accumulation(), acc_o)
+        assert np.array_equal(tgt.rng_state(), rng_o)
+        assert same_bits(tgt.framebuffer(), fb_o)
+
+
 def test_activation_bf16_rounding_is_round_to_nearest_even(device):
     """The radiance cache's MLP rounds its relu'd activations to bf16 with
     the hardware conversion (orn_eval_math op 5). For every non-NaN float it

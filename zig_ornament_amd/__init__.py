@@ -432,7 +432,7 @@ class DeviceScene:
 
 
 def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16,
-                   samples_per_job=16, split_percent=0, node_steps=4, workgroup_pool=0):
+                   samples_per_job=16, split_percent=0, node_steps=4, workgroup_pool=0, cost_order=0):
     t = N.Tuning()
     t.regroup_lanes = int(regroup_lanes)
     t.workgroups_per_cu = int(workgroups_per_cu)
@@ -442,6 +442,7 @@ def _tuning_struct(regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regr
     t.split_percent = int(split_percent)
     t.node_steps = int(node_steps)
     t.workgroup_pool = int(workgroup_pool)
+    t.cost_order = int(cost_order)
     return t
 
 
@@ -449,7 +450,7 @@ def _tuning_dict(t):
     return {"regroup_lanes": t.regroup_lanes, "workgroups_per_cu": t.workgroups_per_cu,
             "bvh_width": t.bvh_width, "leaf_regroup_lanes": t.leaf_regroup_lanes,
             "samples_per_job": t.samples_per_job, "split_percent": t.split_percent,
-            "node_steps": t.node_steps, "workgroup_pool": t.workgroup_pool}
+            "node_steps": t.node_steps, "workgroup_pool": t.workgroup_pool, "cost_order": t.cost_order}
 
 
 class Target:
@@ -475,9 +476,9 @@ class Target:
         check(lib().orn_target_wait(self._h), "orn_target_wait")
 
     def set_tuning(self, regroup_lanes=48, workgroups_per_cu=0, bvh_width=2, leaf_regroup_lanes=16,
-                   samples_per_job=16, split_percent=0, node_steps=4, workgroup_pool=0):
+                   samples_per_job=16, split_percent=0, node_steps=4, workgroup_pool=0, cost_order=0):
         t = _tuning_struct(regroup_lanes, workgroups_per_cu, bvh_width, leaf_regroup_lanes,
-                           samples_per_job, split_percent, node_steps, workgroup_pool)
+                           samples_per_job, split_percent, node_steps, workgroup_pool, cost_order)
         check(lib().orn_target_set_tuning(self._h, C.byref(t)), "orn_target_set_tuning")
 
     def tuning(self):
@@ -514,6 +515,11 @@ class Target:
 
     def rng_state(self):
         return self._read(lib().orn_get_rng_state, np.uint32, 1, "orn_get_rng_state")
+
+    def pixel_costs(self):
+        """Per-pixel GPU time (100 MHz ticks) recorded by the latest render
+        with cost_order 1, row-major (orn_get_pixel_costs)."""
+        return self._read(lib().orn_get_pixel_costs, np.uint32, 1, "orn_get_pixel_costs")
 
     def set_state(self, accumulation=None, rng=None):
         n = self.width * self.height

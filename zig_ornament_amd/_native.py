@@ -65,7 +65,7 @@ class SceneDesc(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("regroup_lanes", u32), ("workgroups_per_cu", u32), ("bvh_width", u32),
                 ("leaf_regroup_lanes", u32), ("samples_per_job", u32), ("split_percent", u32), ("node_steps", u32),
-                ("workgroup_pool", u32)]
+                ("workgroup_pool", u32), ("cost_order", u32)]
 
 
 class RenderSettings(C.Structure):
@@ -134,6 +134,7 @@ DEVICE_API = [
     ("orn_get_framebuffer", st, [vp, vp, sz]),
     ("orn_get_accumulation", st, [vp, vp, sz]),
     ("orn_get_rng_state", st, [vp, vp, sz]),
+    ("orn_get_pixel_costs", st, [vp, vp, sz]),
     ("orn_set_state", st, [vp, vp, vp, sz]),
     ("orn_trace_rays", st, [vp, f32, vp, sz, vp]),
     ("orn_trace_rays_packet", st, [vp, f32, vp, sz, vp]),

@@ -48,6 +48,13 @@ struct orn_target {
     uint32_t grid;     // persistent workgroups of the render kernel
     hipEvent_t ev_start, ev_stop;
     bool timed;
+    // Cost-ordered queues (tuning cost_order), allocated by the first render
+    // that asks for them.
+    uint32_t *cost = nullptr;    // per pixel: GPU time of its jobs in the recording render
+    uint32_t *order = nullptr;   // per band position: the band's tile-pixel queued there
+    uint32_t *counts = nullptr;  // per tile and cost class: pixels, then their prefix over the band
+    uint32_t *totals = nullptr;  // per band and cost class: pixels
+    bool costs_recorded = false; // `cost` holds a whole render's recording
 };
 
 namespace orn_rt {

@@ -58,6 +58,8 @@ struct KernelParams {
     uint32_t epoch;         // tag of this launch in the stage words (single image)
     uint32_t split_pct;     // percentage of each band's pixels that are split
     uint32_t pixels;        // pixels in the pixel buffers (at most 2^28)
+    const uint32_t *order;  // queue position -> tile-pixel of the band, or null: tile order (single image)
+    uint32_t *cost;         // per-pixel job time to record, or null (single image)
 };
 
 // One image of a batch launch. All images of a batch share one device scene
@@ -120,6 +122,8 @@ __device__ __forceinline__ void store_coherent(uint32_t *p, uint32_t v)
 //   a wavefront-scope acquire fence, which emits no instruction but keeps the
 //   compiler from moving the state loads above the stage-word load.
 __device__ __forceinline__ void release_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Low 32 bits of the device's constant 100 MHz clock (wave-uniform).
+__device__ __forceinline__ uint32_t device_ticks() { return static_cast<uint32_t>(__builtin_amdgcn_s_memrealtime()); }
 __device__ __forceinline__ void acquire_order() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
 
 // 16-byte accumulation entries go through a buffer descriptor, whose
@@ -152,6 +156,10 @@ __device__ __forceinline__ void store_coherent(__amdgpu_buffer_rsrc_t r, uint32_
 // stage word says how many jobs of this launch have finished (a lane that
 // draws a job before its predecessor has finished waits for it). A wave
 // whose band is empty moves on to the next band, so uneven bands even out.
+// With a queue order (tuning cost_order), position q of a band's queue is
+// the band's tile-pixel order[q] instead of the q-th: the pixels that cost
+// most in the previous render come first, so the jobs drawn last are cheap
+// and the lanes run out of work nearly together.
 // Within a wave, a lane alternates between traversing its current ray one
 // node per loop iteration and a shading round (hit shading, accumulation,
 // next camera path, next job). A lane whose traversal ended waits; the wave
@@ -218,6 +226,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     uint32_t job = 0;   // index of the lane's job within its pixel
     uint32_t lane_chunks = 1;  // jobs of the lane's pixel (1: a whole pixel)
     uint32_t release_pix = 0, release_tag = 0; // stage word still to be written
+    uint32_t job_t0 = 0;  // device clock when the lane's job began (cost recording)
     uint32_t rng = 0;
     float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float cur = 0.0f;
@@ -270,6 +279,9 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     // also writes the framebuffer (pathtracer.wgsl:81-93), the others
     // release the pixel to its next job.
     auto end_job = [&]() {
+        if (!BATCH && kp.cost)  // the pixel's cost for the next render's queue order
+            __hip_atomic_fetch_add(kp.cost + pix, device_ticks() - job_t0, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         if (job + 1u < lane_chunks) {  // handed on; released at the wave's next flush
             store_coherent(acc_rsrc, pix, acc);
             store_coherent(rng_buf + pix, rng);
@@ -335,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     // per-sample increments reach.
     auto begin_job = [&]() {
         const orn_constant_params &p = params();
+        if (!BATCH && kp.cost) job_t0 = device_ticks();
         if (job == 0u) {  // state left by an earlier launch
             rng = rng_buf[pix];
             acc = acc_buf[pix];
@@ -489,8 +502,16 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
         const uint64_t waiting = lanes_where(mode == kWaiting);
         const uint64_t traversing = lanes_where(mode == kTraverse);
         if (waiting == 0u && traversing == 0u) {
-            if constexpr (POOL) continue;  // the workgroup leaves together, at the loop's top
-            else break;
+            if constexpr (POOL) {
+                // the workgroup leaves together, at the loop's top; a release
+                // this wave still holds is written now, since a lane of
+                // another wave of the workgroup may be waiting for it (and
+                // the workgroup leaves only once that lane is done)
+                flush_releases();
+                continue;
+            } else {
+                break;
+            }
         }
         // a shading round once `regroup` lanes wait, or once fewer than half
         // as many lanes still traverse as wait (the wave would otherwise step
@@ -564,7 +585,10 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                         c = (q - bulk) / band_split;
                         qp = bulk + (q - bulk - c * band_split);
                     }
-                    const uint32_t within = qp % (kTile * kTile);
+                    // position qp of the band's queue -> tile-pixel of the band
+                    const uint32_t qpx =
+                        (!BATCH && kp.order) ? kp.order[band * kp.tiles_per_xcd * (kTile * kTile) + qp] : qp;
+                    const uint32_t within = qpx % (kTile * kTile);
                     uint32_t tile, tiles_x, w, h, chunks;
                     if constexpr (BATCH) {
                         const uint2 e = work->tiles[work->band_begin[band] + qp / (kTile * kTile)];
@@ -575,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                         h = items[item].p.height;
                         chunks = items[item].chunks;
                     } else {
-                        tile = band * kp.tiles_per_xcd + qp / (kTile * kTile);
+                        tile = band * kp.tiles_per_xcd + qpx / (kTile * kTile);
                         tiles_x = kp.tiles_x;
                         w = kp.p.width;
                         h = kp.p.height;
@@ -619,6 +643,127 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
 __global__ void reset_queues_kernel(uint32_t *q)
 {
     if (threadIdx.x < kXcds) q[threadIdx.x] = 0u;
+}
+
+// Queue order by cost (tuning cost_order): a stable counting sort of each
+// band's tile-pixels on their cost class, highest class first. One workgroup
+// per tile counts its pixels per class (order_count_kernel); one wave per
+// band and class turns the counts into prefixes over the band's tiles
+// (order_scan_kernel); one workgroup per tile places its pixels
+// (order_place_kernel), which also clears the costs for the render that
+// records the next ones. Pixels outside the image (partial edge tiles) take
+// class 0 and go last; the render skips them as before.
+constexpr uint32_t kCostClasses = 128;
+
+// Quarter-octave class of a cost: 4 * floor(log2(c + 1)) plus the two bits
+// below the leading one of c + 1; non-decreasing in c, below 128.
+__device__ __forceinline__ uint32_t cost_class(uint32_t c)
+{
+    const uint32_t v = c == 0xffffffffu ? c : c + 1u;
+    const uint32_t e = 31u - static_cast<uint32_t>(__clz(v));
+    const uint32_t f = e >= 2u ? (v >> (e - 2u)) & 3u : (v << (2u - e)) & 3u;
+    return 4u * e + f;
+}
+
+struct OrderShape {
+    uint32_t tiles_x, tiles, tiles_per_xcd, width, height;
+};
+
+// cost class of pixel i (0..255) of a tile; 0 outside the image
+__device__ __forceinline__ uint32_t tile_pixel_class(const uint32_t *cost, const OrderShape &o, uint32_t tile,
+                                                     uint32_t i, uint32_t *pixel)
+{
+    const uint32_t x = (tile % o.tiles_x) * kTile + i % kTile;
+    const uint32_t y = (tile / o.tiles_x) * kTile + i / kTile;
+    if (x >= o.width || y >= o.height) {
+        *pixel = 0xffffffffu;
+        return 0u;
+    }
+    *pixel = y * o.width + x;
+    return cost_class(cost[*pixel]);
+}
+
+__global__ __launch_bounds__(kBlock) void order_count_kernel(const uint32_t *__restrict__ cost, OrderShape o,
+                                                             uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t hist[kCostClasses];
+    if (threadIdx.x < kCostClasses) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t pixel;
+    atomicAdd(&hist[tile_pixel_class(cost, o, blockIdx.x, threadIdx.x, &pixel)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kCostClasses) counts[blockIdx.x * kCostClasses + threadIdx.x] = hist[threadIdx.x];
+}
+
+// grid: kXcds * kCostClasses waves; wave w scans class w % kCostClasses over
+// the tiles of band w / kCostClasses, in place (exclusive), and stores the
+// band's total of the class.
+__global__ __launch_bounds__(kBlock) void order_scan_kernel(uint32_t *__restrict__ counts, OrderShape o,
+                                                            uint32_t *__restrict__ totals)
+{
+    const uint32_t w = (blockIdx.x * kBlock + threadIdx.x) / 64u;
+    const uint32_t lane = threadIdx.x % 64u;
+    const uint32_t band = w / kCostClasses, cls = w % kCostClasses;
+    const uint32_t first = band * o.tiles_per_xcd;
+    const uint32_t end = first >= o.tiles ? first : min(o.tiles, first + o.tiles_per_xcd);
+    uint32_t carry = 0;
+    for (uint32_t t0 = first; t0 < end; t0 += 64u) {
+        const uint32_t t = t0 + lane;
+        const uint32_t v = t < end ? counts[t * kCostClasses + cls] : 0u;
+        uint32_t incl = v;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t up = __shfl_up(incl, d);
+            if (lane >= d) incl += up;
+        }
+        if (t < end) counts[t * kCostClasses + cls] = carry + incl - v;
+        carry += __shfl(incl, 63);
+    }
+    if (lane == 0u) totals[band * kCostClasses + cls] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void order_place_kernel(uint32_t *__restrict__ cost, OrderShape o,
+                                                             const uint32_t *__restrict__ prefix,
+                                                             const uint32_t *__restrict__ totals,
+                                                             uint32_t *__restrict__ order)
+{
+    __shared__ uint32_t base[kCostClasses];  // band positions taken by the classes above
+    __shared__ uint32_t wave_count[kBlock / 64u][kCostClasses];
+    const uint32_t tile = blockIdx.x, band = tile / o.tiles_per_xcd;
+    const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    for (uint32_t i = threadIdx.x; i < (kBlock / 64u) * kCostClasses; i += kBlock) (&wave_count[0][0])[i] = 0u;
+    if (threadIdx.x < kCostClasses) {
+        // exclusive scan over the classes from the highest down (two waves)
+        const uint32_t c = kCostClasses - 1u - threadIdx.x;
+        const uint32_t v = totals[band * kCostClasses + c];
+        uint32_t incl = v;
+        for (uint32_t d = 1; d < 64u; d <<= 1) {
+            const uint32_t up = __shfl_up(incl, d);
+            if (lane >= d) incl += up;
+        }
+        base[c] = incl - v;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64u && threadIdx.x < kCostClasses)  // the second wave adds the first one's total
+        base[kCostClasses - 1u - threadIdx.x] += base[kCostClasses - 64u] + totals[band * kCostClasses + kCostClasses - 64u];
+    uint32_t pixel;
+    const uint32_t cls = tile_pixel_class(cost, o, tile, threadIdx.x, &pixel);
+    if (pixel != 0xffffffffu) cost[pixel] = 0u;
+    // rank among the wave's lanes of the same class, in lane order
+    uint32_t rank = 0;
+    uint64_t todo = lanes_where(true);
+    while (todo != 0u) {
+        const uint32_t c = static_cast<uint32_t>(
+            __builtin_amdgcn_readlane(static_cast<int>(cls), __ffsll(static_cast<unsigned long long>(todo)) - 1));
+        const uint64_t m = lanes_where(cls == c);
+        if (cls == c) rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+        if (lane == 0u) wave_count[wave][c] = static_cast<uint32_t>(__popcll(m));
+        todo &= ~m;
+    }
+    __syncthreads();
+    for (uint32_t w = 0; w < wave; w++) rank += wave_count[w][cls];
+    const uint32_t band_first = band * o.tiles_per_xcd * (kTile * kTile);
+    order[band_first + base[cls] + prefix[tile * kCostClasses + cls] + rank] =
+        (tile - band * o.tiles_per_xcd) * (kTile * kTile) + threadIdx.x;
 }
 
 template <uint32_t W>
@@ -955,7 +1100,7 @@ bool tuning_valid(const orn_tuning *t)
     return t && t->regroup_lanes >= 1 && t->regroup_lanes <= 64 && t->workgroups_per_cu <= 64 &&
            (t->bvh_width == 2u || t->bvh_width == 4u) && t->leaf_regroup_lanes >= 1 &&
            t->leaf_regroup_lanes <= 64 && t->samples_per_job <= 0xffffu && t->split_percent <= 100u &&
-           t->node_steps <= 16u && t->workgroup_pool <= 1u;
+           t->node_steps <= 16u && t->workgroup_pool <= 1u && t->cost_order <= 1u;
 }
 
 // node_steps 0 stands for the default
@@ -1422,6 +1567,10 @@ orn_status orn_target_destroy(orn_target *t)
     if (t->rng) (void)hipFree(t->rng);
     if (t->queues) (void)hipFree(t->queues);
     if (t->stage) (void)hipFree(t->stage);
+    if (t->cost) (void)hipFree(t->cost);
+    if (t->order) (void)hipFree(t->order);
+    if (t->counts) (void)hipFree(t->counts);
+    if (t->totals) (void)hipFree(t->totals);
     if (t->ev_start) (void)hipEventDestroy(t->ev_start);
     if (t->ev_stop) (void)hipEventDestroy(t->ev_stop);
     orn_device_scene *s = t->scene;
@@ -1440,7 +1589,8 @@ namespace {
 // was last cleared). Allocates nothing.
 hipError_t enqueue_render(orn_device_scene *s, const orn_constant_params *params, uint32_t iterations,
                           const orn_tuning &tuning, float4 *acc, uint32_t *rng, float4 *fb, uint32_t *stage,
-                          uint32_t *queues, uint32_t epoch, hipStream_t st)
+                          uint32_t *queues, uint32_t epoch, hipStream_t st, const uint32_t *order = nullptr,
+                          uint32_t *cost = nullptr)
 {
     KernelParams kp{};
     kp.p = *params;
@@ -1461,10 +1611,52 @@ hipError_t enqueue_render(orn_device_scene *s, const orn_constant_params *params
     kp.epoch = epoch;
     kp.split_pct = tuning.split_percent;
     kp.pixels = params->width * params->height;
+    kp.order = order;
+    kp.cost = cost;
     LaunchArgs a{kp, nullptr, nullptr, acc, rng, fb, stage, queues, kp.tiles, tuning.workgroup_pool != 0u};
     reset_queues_kernel<<<1, 64, 0, st>>>(queues);
     return launch_render<false>(s->features, width, tuning.workgroups_per_cu,
                                 static_cast<uint32_t>(s->ctx->compute_units), lds, st, ds, a);
+}
+
+// Buffers of a target's cost-ordered queues, allocated on first use; the
+// costs start at zero.
+hipError_t ensure_cost_order(orn_target *t, hipStream_t st)
+{
+    if (t->cost) return hipSuccess;
+    const uint32_t n = t->width * t->height;
+    const uint32_t tiles = blocks_for(t->width, kTile) * blocks_for(t->height, kTile);
+    const uint32_t positions = blocks_for(tiles, kXcds) * kXcds * (kTile * kTile);
+    hipError_t e = hipMalloc(&t->cost, size_t(n) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&t->order, size_t(positions) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&t->counts, size_t(tiles) * kCostClasses * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&t->totals, size_t(kXcds) * kCostClasses * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        (void)hipFree(t->cost);
+        (void)hipFree(t->order);
+        (void)hipFree(t->counts);
+        (void)hipFree(t->totals);
+        t->cost = t->order = t->counts = t->totals = nullptr;
+        return e;
+    }
+    fill_u32_kernel<<<blocks_for(n, 256), 256, 0, st>>>(t->cost, n, 0u);
+    t->costs_recorded = false;
+    return hipGetLastError();
+}
+
+// The queue order of a target's next render from the costs its previous
+// recording render left (and the costs cleared for the next recording).
+void enqueue_queue_order(orn_target *t, hipStream_t st)
+{
+    OrderShape o;
+    o.tiles_x = blocks_for(t->width, kTile);
+    o.tiles = o.tiles_x * blocks_for(t->height, kTile);
+    o.tiles_per_xcd = blocks_for(o.tiles, kXcds);
+    o.width = t->width;
+    o.height = t->height;
+    order_count_kernel<<<o.tiles, kBlock, 0, st>>>(t->cost, o, t->counts);
+    order_scan_kernel<<<kXcds * kCostClasses * 64u / kBlock, kBlock, 0, st>>>(t->counts, o, t->totals);
+    order_place_kernel<<<o.tiles, kBlock, 0, st>>>(t->cost, o, t->counts, t->totals, t->order);
 }
 
 }  // namespace
@@ -1484,11 +1676,19 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
         fill_u32_kernel<<<blocks_for(n, 256), 256, 0, st>>>(t->stage, n, 0u);
         t->epoch = 1;
     }
+    const bool by_cost = t->tuning.cost_order != 0u;
+    if (by_cost) ORN_TRY(ensure_cost_order(t, st));
     ORN_TRY(hipEventRecord(t->ev_start, st));
+    const uint32_t *order = nullptr;
+    if (by_cost && t->costs_recorded) {
+        enqueue_queue_order(t, st);
+        order = t->order;
+    }
     ORN_TRY(enqueue_render(s, params, iterations, t->tuning, t->acc, t->rng, t->fb, t->stage, t->queues, t->epoch,
-                           st));
+                           st, order, by_cost ? t->cost : nullptr));
     ORN_TRY(hipEventRecord(t->ev_stop, st));
     t->timed = true;
+    if (by_cost) t->costs_recorded = true;
     return ORN_OK;
 }
 
@@ -1589,6 +1789,12 @@ orn_status orn_get_accumulation(orn_target *t, float *dst, size_t pixels)
 orn_status orn_get_rng_state(orn_target *t, uint32_t *dst, size_t pixels)
 {
     return readback(t, dst, t ? t->rng : nullptr, sizeof(uint32_t), pixels);
+}
+
+orn_status orn_get_pixel_costs(orn_target *t, uint32_t *dst, size_t pixels)
+{
+    if (t && !t->costs_recorded) return ORN_ERROR_INVALID_VALUE;
+    return readback(t, dst, t ? t->cost : nullptr, sizeof(uint32_t), pixels);
 }
 
 orn_status orn_set_state(orn_target *t, const float *accumulation, const uint32_t *rng, size_t pixels)
@@ -2308,6 +2514,7 @@ bool cached_tuning(const char *path, const std::string &key, orn_tuning *t, floa
             else if (f.first == "split_percent") r.split_percent = v;
             else if (f.first == "node_steps") r.node_steps = v;
             else if (f.first == "workgroup_pool") r.workgroup_pool = v;
+            else if (f.first == "cost_order") r.cost_order = v;
             else if (f.first == "ms") *ms = static_cast<float>(f.second);
         }
         if (!(ok[0] && ok[1] && ok[2] && ok[3])) return false;
@@ -2331,6 +2538,7 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
         {"split_percent", t.split_percent},
         {"node_steps", t.node_steps},
         {"workgroup_pool", t.workgroup_pool},
+        {"cost_order", t.cost_order},
         {"ms", ms}};
     bool found = false;
     for (auto &e : c)
@@ -2346,7 +2554,8 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
 // tuning and returns its GPU time. Each candidate keeps the faster of two
 // renders.
 template <class Run>
-orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, orn_tuning *best, float *best_ms)
+orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, bool queue_order, Run run, orn_tuning *best,
+                              float *best_ms)
 {
     std::vector<std::pair<orn_tuning, float>> seen;
     auto measure = [&](const orn_tuning &t, float *ms) -> orn_status {
@@ -2374,6 +2583,7 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
     static const uint32_t split[] = {25, 50, 100};
     static const uint32_t job[] = {2, 4, 8, 16, 32};
     static const uint32_t steps[] = {1, 2, 4, 6, 8, 12};
+    static const uint32_t off_on[] = {0, 1};
     auto sweep = [&](uint32_t orn_tuning::*field, const uint32_t *vals, size_t n) -> orn_status {
         for (size_t i = 0; i < n; i++) {
             orn_tuning t = cur;
@@ -2412,6 +2622,7 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, 
                 }
             }
         if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
+        if (queue_order && (st = sweep(&orn_tuning::cost_order, off_on, 2)) != ORN_OK) return st;
         if (!(cur_ms < before)) break;
     }
     // Then the settings between the coarse candidates around the best one:
@@ -2507,7 +2718,7 @@ orn_status orn_target_autotune(orn_target *t, const orn_constant_params *params,
             const hipError_t e = saved.restore(t->acc, t->rng, t->fb);
             return r != ORN_OK ? r : from_hip(e);
         };
-        const orn_status r = coordinate_descent(start, flags, run, &best, &best_ms);
+        const orn_status r = coordinate_descent(start, flags, true, run, &best, &best_ms);
         t->tuning = r == ORN_OK ? best : start;
         if (r != ORN_OK) return r;
         store_tuning(cache_path, key, best, best_ms);
@@ -2549,7 +2760,7 @@ orn_status orn_batch_autotune(orn_batch *b, const orn_constant_params *params, c
             const hipError_t e = saved.restore(b->acc, b->rng, b->fb);
             return r != ORN_OK ? r : from_hip(e);
         };
-        const orn_status r = coordinate_descent(start, flags, run, &best, &best_ms);
+        const orn_status r = coordinate_descent(start, flags, false, run, &best, &best_ms);
         (void)orn_batch_set_tuning(b, r == ORN_OK ? &best : &start);
         if (r != ORN_OK) return r;
         store_tuning(cache_path, key, best, best_ms);
