@@ -136,7 +136,8 @@ def test_device_errors(device):
 
 def test_tuning_validation_and_defaults(device):
     """orn_target_set_tuning rejects out-of-range fields (node_steps above
-    16, workgroup_pool above 1) and stores node_steps 0 as its default."""
+    16, workgroup_pool or cost_order above 1) and stores node_steps 0 as its
+    default."""
     import ctypes as C
     from zig_ornament_amd import _native as N
     spec = W.cornell(width=16, height=16, spp=1)
@@ -156,7 +157,9 @@ def test_tuning_validation_and_defaults(device):
     assert N.lib().orn_target_get_tuning(t._h, C.byref(raw)) == 0
     raw.workgroup_pool = 2
     assert N.lib().orn_target_set_tuning(t._h, C.byref(raw)) == 1  # ORN_ERROR_INVALID_VALUE
-    assert t.tuning()["node_steps"] == 16
+    raw.workgroup_pool, raw.cost_order = 0, 2
+    assert N.lib().orn_target_set_tuning(t._h, C.byref(raw)) == 1
+    assert t.tuning()["node_steps"] == 16 and t.tuning()["cost_order"] == 0
 
 
 def test_fractional_iteration_counter_matches_oracle(device):
