@@ -256,14 +256,13 @@ orn_status orn_target_wait(orn_target *target);
  *   wait. It applies to scenes of spheres only with the BVH2 and is ignored
  *   elsewhere (default 0).
  * cost_order (0 or 1): 1 queues each XCD's pixels most expensive first, by
- *   what they cost in the target's previous render with cost_order 1 (GPU
- *   time of their jobs, in quarter-octave classes; tile order within a
- *   class), so that the cheapest pixels are the last ones drawn and the end
- *   of the render, where lanes run out of work, is short (longest processing
- *   time first). The first such render of a target queues in tile order and
- *   records the costs; every such render records them for the next. It
- *   applies to orn_render and orn_render_async and is ignored by batches and
- *   orn_render_noalloc (default 0). */
+ *   what they cost in the target's (or batch's) previous render with
+ *   cost_order 1 (GPU time of their jobs, in quarter-octave classes; tile
+ *   order within a class), so that the cheapest pixels are the last ones
+ *   drawn and the end of the render, where lanes run out of work, is short
+ *   (longest processing time first). The first such render queues in tile
+ *   order and records the costs; every such render records them for the
+ *   next. orn_render_noalloc ignores it (default 0). */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;
@@ -372,8 +371,8 @@ orn_status orn_batch_set_state(orn_batch *batch, size_t item, const float *accum
 /* Times renders of `iterations` samples per pixel (a few suffice) under
  * candidate tunings (coordinate descent over regroup_lanes,
  * leaf_regroup_lanes, node_steps, workgroups_per_cu, split_percent,
- * samples_per_job, cost_order (targets only) and, with ORN_TUNE_ALLOW_BVH4,
- * bvh_width; then the neighbours of the best regroup_lanes,
+ * samples_per_job, cost_order and, with ORN_TUNE_ALLOW_BVH4, bvh_width;
+ * then the neighbours of the best regroup_lanes,
  * leaf_regroup_lanes and node_steps),
  * sets the fastest on the target and returns it with its time.
  * The target's accumulation, RNG state and framebuffer are left as they

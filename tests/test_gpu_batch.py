@@ -61,6 +61,34 @@ def test_batch_matches_separate_targets_over_graph_replays(device):
     batch.destroy()
 
 
+@pytest.mark.parametrize("tuning", [dict(cost_order=1),
+                                    dict(cost_order=1, samples_per_job=2, split_percent=50, regroup_lanes=16)])
+def test_cost_ordered_batch_matches_separate_targets(device, tuning):
+    """With cost_order 1 the batch graph orders each band's queue by the
+    pixel costs its previous replay recorded (the first replay has none and
+    keeps the work-list order). Three replays in a row must each leave every
+    image as a separate target leaves it."""
+    specs = mixed_specs()
+    arrays, batch = batch_of(device, specs)
+    batch.set_tuning(**tuning)
+    targets = []
+    for i, s in enumerate(specs):
+        ds = zo.DeviceScene(device, arrays[i])
+        targets.append((ds, zo.Target(ds, s.width, s.height, s.seed)))
+    for k in range(3):
+        print(f"[batch] cost-ordered replay {k}, {tuning}", flush=True)
+        params = [zo_params(s, current_iteration=float(k * max(s.spp, 1))) for s in specs]
+        iters = [s.spp for s in specs]
+        batch.render(params, iters)
+        for i, s in enumerate(specs):
+            t = targets[i][1]
+            t.render(params[i], iters[i])
+            assert same_bits(batch.accumulation(i), t.accumulation()), (k, s.name)
+            assert np.array_equal(batch.rng_state(i), t.rng_state()), (k, s.name)
+            assert same_bits(batch.framebuffer(i), t.framebuffer()), (k, s.name)
+    batch.destroy()
+
+
 def test_batch_matches_oracle_image(device, oracle):
     specs = [W.instanced_scene(2000, 24, 2, seed=21), W.cornell(width=20, height=20, spp=2)]
     arrays, batch = batch_of(device, specs)

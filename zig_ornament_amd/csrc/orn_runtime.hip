@@ -58,8 +58,8 @@ struct KernelParams {
     uint32_t epoch;         // tag of this launch in the stage words (single image)
     uint32_t split_pct;     // percentage of each band's pixels that are split
     uint32_t pixels;        // pixels in the pixel buffers (at most 2^28)
-    const uint32_t *order;  // queue position -> tile-pixel of the band, or null: tile order (single image)
-    uint32_t *cost;         // per-pixel job time to record, or null (single image)
+    const uint32_t *order;  // queue position -> tile-pixel of the band, or null: tile order
+    uint32_t *cost;         // per-pixel job time to record, or null
 };
 
 // One image of a batch launch. All images of a batch share one device scene
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     // also writes the framebuffer (pathtracer.wgsl:81-93), the others
     // release the pixel to its next job.
     auto end_job = [&]() {
-        if (!BATCH && kp.cost)  // the pixel's cost for the next render's queue order
+        if (kp.cost)  // the pixel's cost for the next render's queue order
             __hip_atomic_fetch_add(kp.cost + pix, device_ticks() - job_t0, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         if (job + 1u < lane_chunks) {  // handed on; released at the wave's next flush
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
     // per-sample increments reach.
     auto begin_job = [&]() {
         const orn_constant_params &p = params();
-        if (!BATCH && kp.cost) job_t0 = device_ticks();
+        if (kp.cost) job_t0 = device_ticks();
         if (job == 0u) {  // state left by an earlier launch
             rng = rng_buf[pix];
             acc = acc_buf[pix];
@@ -587,11 +587,13 @@ __global__ __launch_bounds__(kBlock) void render_kernel(DevScene sc, KernelParam
                     }
                     // position qp of the band's queue -> tile-pixel of the band
                     const uint32_t qpx =
-                        (!BATCH && kp.order) ? kp.order[band * kp.tiles_per_xcd * (kTile * kTile) + qp] : qp;
+                        kp.order ? kp.order[(BATCH ? work->band_begin[band] : band * kp.tiles_per_xcd) *
+                                                (kTile * kTile) + qp]
+                                 : qp;
                     const uint32_t within = qpx % (kTile * kTile);
                     uint32_t tile, tiles_x, w, h, chunks;
                     if constexpr (BATCH) {
-                        const uint2 e = work->tiles[work->band_begin[band] + qp / (kTile * kTile)];
+                        const uint2 e = work->tiles[work->band_begin[band] + qpx / (kTile * kTile)];
                         item = e.x;
                         tile = e.y;
                         tiles_x = items[item].tiles_x;
@@ -651,8 +653,9 @@ __global__ void reset_queues_kernel(uint32_t *q)
 // band and class turns the counts into prefixes over the band's tiles
 // (order_scan_kernel); one workgroup per tile places its pixels
 // (order_place_kernel), which also clears the costs for the render that
-// records the next ones. Pixels outside the image (partial edge tiles) take
-// class 0 and go last; the render skips them as before.
+// records the next ones. Pixels outside an image (partial edge tiles) take
+// class 0 and go last; the render skips them as before. With every cost zero
+// (nothing recorded yet) the order is the tile order.
 constexpr uint32_t kCostClasses = 128;
 
 // Quarter-octave class of a cost: 4 * floor(log2(c + 1)) plus the two bits
@@ -665,21 +668,53 @@ __device__ __forceinline__ uint32_t cost_class(uint32_t c)
     return 4u * e + f;
 }
 
+// The tiles a launch queues: one image's tiles in row-major order, cut into
+// kXcds bands of tiles_per_xcd (work == nullptr), or a batch's work list.
 struct OrderShape {
+    const BatchTiles *work;
+    const BatchItem *items;
     uint32_t tiles_x, tiles, tiles_per_xcd, width, height;
 };
 
-// cost class of pixel i (0..255) of a tile; 0 outside the image
-__device__ __forceinline__ uint32_t tile_pixel_class(const uint32_t *cost, const OrderShape &o, uint32_t tile,
+// first tile and tile count of band b
+__device__ __forceinline__ uint2 band_range(const OrderShape &o, uint32_t b)
+{
+    if (o.work) return make_uint2(o.work->band_begin[b], o.work->band_tiles[b]);
+    const uint32_t first = b * o.tiles_per_xcd;
+    return make_uint2(first, first >= o.tiles ? 0u : min(o.tiles_per_xcd, o.tiles - first));
+}
+
+// band holding tile t
+__device__ __forceinline__ uint32_t band_of(const OrderShape &o, uint32_t t)
+{
+    if (!o.work) return t / o.tiles_per_xcd;
+    uint32_t b = 0;
+    while (b + 1u < kXcds && t >= o.work->band_begin[b + 1u]) b++;
+    return b;
+}
+
+// cost class of pixel i (0..255) of tile t, and its index in the pixel
+// buffers (0xffffffff outside the image: class 0)
+__device__ __forceinline__ uint32_t tile_pixel_class(const uint32_t *cost, const OrderShape &o, uint32_t t,
                                                      uint32_t i, uint32_t *pixel)
 {
-    const uint32_t x = (tile % o.tiles_x) * kTile + i % kTile;
-    const uint32_t y = (tile / o.tiles_x) * kTile + i / kTile;
-    if (x >= o.width || y >= o.height) {
+    uint32_t tile = t, tiles_x = o.tiles_x, w = o.width, h = o.height, base = 0;
+    if (o.work) {
+        const uint2 e = o.work->tiles[t];
+        const BatchItem &it = o.items[e.x];
+        tile = e.y;
+        tiles_x = it.tiles_x;
+        w = it.p.width;
+        h = it.p.height;
+        base = it.px_base;
+    }
+    const uint32_t x = (tile % tiles_x) * kTile + i % kTile;
+    const uint32_t y = (tile / tiles_x) * kTile + i / kTile;
+    if (x >= w || y >= h) {
         *pixel = 0xffffffffu;
         return 0u;
     }
-    *pixel = y * o.width + x;
+    *pixel = base + y * w + x;
     return cost_class(cost[*pixel]);
 }
 
@@ -704,10 +739,10 @@ __global__ __launch_bounds__(kBlock) void order_scan_kernel(uint32_t *__restrict
     const uint32_t w = (blockIdx.x * kBlock + threadIdx.x) / 64u;
     const uint32_t lane = threadIdx.x % 64u;
     const uint32_t band = w / kCostClasses, cls = w % kCostClasses;
-    const uint32_t first = band * o.tiles_per_xcd;
-    const uint32_t end = first >= o.tiles ? first : min(o.tiles, first + o.tiles_per_xcd);
+    const uint2 r = band_range(o, band);
+    const uint32_t end = r.x + r.y;
     uint32_t carry = 0;
-    for (uint32_t t0 = first; t0 < end; t0 += 64u) {
+    for (uint32_t t0 = r.x; t0 < end; t0 += 64u) {
         const uint32_t t = t0 + lane;
         const uint32_t v = t < end ? counts[t * kCostClasses + cls] : 0u;
         uint32_t incl = v;
@@ -721,6 +756,8 @@ __global__ __launch_bounds__(kBlock) void order_scan_kernel(uint32_t *__restrict
     if (lane == 0u) totals[band * kCostClasses + cls] = carry;
 }
 
+// order[first tile of the band * 256 + position] = the band's tile-pixel
+// (tile within the band * 256 + pixel within the tile) queued there
 __global__ __launch_bounds__(kBlock) void order_place_kernel(uint32_t *__restrict__ cost, OrderShape o,
                                                              const uint32_t *__restrict__ prefix,
                                                              const uint32_t *__restrict__ totals,
@@ -728,7 +765,8 @@ __global__ __launch_bounds__(kBlock) void order_place_kernel(uint32_t *__restric
 {
     __shared__ uint32_t base[kCostClasses];  // band positions taken by the classes above
     __shared__ uint32_t wave_count[kBlock / 64u][kCostClasses];
-    const uint32_t tile = blockIdx.x, band = tile / o.tiles_per_xcd;
+    const uint32_t tile = blockIdx.x, band = band_of(o, tile);
+    const uint32_t first = band_range(o, band).x;
     const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
     for (uint32_t i = threadIdx.x; i < (kBlock / 64u) * kCostClasses; i += kBlock) (&wave_count[0][0])[i] = 0u;
     if (threadIdx.x < kCostClasses) {
@@ -761,9 +799,17 @@ __global__ __launch_bounds__(kBlock) void order_place_kernel(uint32_t *__restric
     }
     __syncthreads();
     for (uint32_t w = 0; w < wave; w++) rank += wave_count[w][cls];
-    const uint32_t band_first = band * o.tiles_per_xcd * (kTile * kTile);
-    order[band_first + base[cls] + prefix[tile * kCostClasses + cls] + rank] =
-        (tile - band * o.tiles_per_xcd) * (kTile * kTile) + threadIdx.x;
+    order[first * (kTile * kTile) + base[cls] + prefix[tile * kCostClasses + cls] + rank] =
+        (tile - first) * (kTile * kTile) + threadIdx.x;
+}
+
+// The three kernels over `tiles` tiles on `st`.
+void enqueue_order_kernels(const OrderShape &o, uint32_t tiles, uint32_t *cost, uint32_t *counts, uint32_t *totals,
+                           uint32_t *order, hipStream_t st)
+{
+    order_count_kernel<<<tiles, kBlock, 0, st>>>(cost, o, counts);
+    order_scan_kernel<<<kXcds * kCostClasses * 64u / kBlock, kBlock, 0, st>>>(counts, o, totals);
+    order_place_kernel<<<tiles, kBlock, 0, st>>>(cost, o, counts, totals, order);
 }
 
 template <uint32_t W>
@@ -1648,15 +1694,13 @@ hipError_t ensure_cost_order(orn_target *t, hipStream_t st)
 // recording render left (and the costs cleared for the next recording).
 void enqueue_queue_order(orn_target *t, hipStream_t st)
 {
-    OrderShape o;
+    OrderShape o{};
     o.tiles_x = blocks_for(t->width, kTile);
     o.tiles = o.tiles_x * blocks_for(t->height, kTile);
     o.tiles_per_xcd = blocks_for(o.tiles, kXcds);
     o.width = t->width;
     o.height = t->height;
-    order_count_kernel<<<o.tiles, kBlock, 0, st>>>(t->cost, o, t->counts);
-    order_scan_kernel<<<kXcds * kCostClasses * 64u / kBlock, kBlock, 0, st>>>(t->counts, o, t->totals);
-    order_place_kernel<<<o.tiles, kBlock, 0, st>>>(t->cost, o, t->counts, t->totals, t->order);
+    enqueue_order_kernels(o, o.tiles, t->cost, t->counts, t->totals, t->order, st);
 }
 
 }  // namespace
@@ -2048,6 +2092,12 @@ struct orn_batch {
     bool graph_ok;
     hipEvent_t ev_start, ev_stop;
     bool timed;
+    // cost-ordered queues (tuning cost_order), allocated by the first capture
+    // that asks for them
+    uint32_t *cost = nullptr;    // per pooled pixel: GPU time of its jobs in the latest render
+    uint32_t *order = nullptr;   // per work-list position: the band's tile-pixel queued there
+    uint32_t *counts = nullptr;  // per work-list tile and cost class
+    uint32_t *totals = nullptr;  // per band and cost class
 };
 
 namespace {
@@ -2063,6 +2113,10 @@ void free_batch(orn_batch *b)
     if (b->h_work) (void)hipHostFree(b->h_work);
     if (b->d_items) (void)hipFree(b->d_items);
     if (b->d_work) (void)hipFree(b->d_work);
+    if (b->cost) (void)hipFree(b->cost);
+    if (b->order) (void)hipFree(b->order);
+    if (b->counts) (void)hipFree(b->counts);
+    if (b->totals) (void)hipFree(b->totals);
     if (b->fb) (void)hipFree(b->fb);
     if (b->acc) (void)hipFree(b->acc);
     if (b->rng) (void)hipFree(b->rng);
@@ -2100,7 +2154,31 @@ void build_work(orn_batch *b, const std::vector<uint64_t> &cost)
     b->work_cost = cost;
 }
 
-// Captures [item table copy, queue reset, render launch] as one graph.
+// Buffers of a batch's cost-ordered queues, allocated on first use; the
+// costs start at zero (so the first render queues in work-list order).
+hipError_t ensure_batch_cost_order(orn_batch *b, hipStream_t st)
+{
+    if (b->cost) return hipSuccess;
+    const size_t positions = size_t(b->total_tiles) * (kTile * kTile);
+    hipError_t e = hipMalloc(&b->cost, size_t(b->total_px) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->order, positions * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->counts, size_t(b->total_tiles) * kCostClasses * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->totals, size_t(kXcds) * kCostClasses * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        (void)hipFree(b->cost);
+        (void)hipFree(b->order);
+        (void)hipFree(b->counts);
+        (void)hipFree(b->totals);
+        b->cost = b->order = b->counts = b->totals = nullptr;
+        return e;
+    }
+    fill_u32_kernel<<<blocks_for(b->total_px, 256), 256, 0, st>>>(b->cost, b->total_px, 0u);
+    return hipGetLastError();
+}
+
+// Captures [item table copy, queue reset, render launch] as one graph; with
+// tuning cost_order, the queue-order kernels run between the copy and the
+// launch, on the costs the previous replay recorded.
 orn_status capture_batch(orn_batch *b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -2120,9 +2198,22 @@ orn_status capture_batch(orn_batch *b)
     kp.node_steps = b->tuning.node_steps;
     kp.split_pct = b->tuning.split_percent;
     kp.pixels = b->total_px;
+    const bool by_cost = b->tuning.cost_order != 0u;
+    if (by_cost) {
+        ORN_TRY(ensure_batch_cost_order(b, st));
+        kp.order = b->order;
+        kp.cost = b->cost;
+    }
     LaunchArgs a{kp, b->d_items, b->d_work, b->acc, b->rng, b->fb, b->stage, b->queues, b->total_tiles};
     ORN_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     hipError_t e = hipMemcpyAsync(b->d_items, b->h_items, b->n * sizeof(BatchItem), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && by_cost) {
+        OrderShape o{};
+        o.work = b->d_work;
+        o.items = b->d_items;
+        enqueue_order_kernels(o, b->total_tiles, b->cost, b->counts, b->totals, b->order, st);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) {
         reset_queues_kernel<<<1, 64, 0, st>>>(b->queues);
         e = hipGetLastError();
@@ -2554,8 +2645,7 @@ void store_tuning(const char *path, const std::string &key, const orn_tuning &t,
 // tuning and returns its GPU time. Each candidate keeps the faster of two
 // renders.
 template <class Run>
-orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, bool queue_order, Run run, orn_tuning *best,
-                              float *best_ms)
+orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, Run run, orn_tuning *best, float *best_ms)
 {
     std::vector<std::pair<orn_tuning, float>> seen;
     auto measure = [&](const orn_tuning &t, float *ms) -> orn_status {
@@ -2622,7 +2712,7 @@ orn_status coordinate_descent(const orn_tuning &start, uint32_t flags, bool queu
                 }
             }
         if ((flags & ORN_TUNE_ALLOW_BVH4) && (st = sweep(&orn_tuning::bvh_width, width, 2)) != ORN_OK) return st;
-        if (queue_order && (st = sweep(&orn_tuning::cost_order, off_on, 2)) != ORN_OK) return st;
+        if ((st = sweep(&orn_tuning::cost_order, off_on, 2)) != ORN_OK) return st;
         if (!(cur_ms < before)) break;
     }
     // Then the settings between the coarse candidates around the best one:
@@ -2718,7 +2808,7 @@ orn_status orn_target_autotune(orn_target *t, const orn_constant_params *params,
             const hipError_t e = saved.restore(t->acc, t->rng, t->fb);
             return r != ORN_OK ? r : from_hip(e);
         };
-        const orn_status r = coordinate_descent(start, flags, true, run, &best, &best_ms);
+        const orn_status r = coordinate_descent(start, flags, run, &best, &best_ms);
         t->tuning = r == ORN_OK ? best : start;
         if (r != ORN_OK) return r;
         store_tuning(cache_path, key, best, best_ms);
@@ -2760,7 +2850,7 @@ orn_status orn_batch_autotune(orn_batch *b, const orn_constant_params *params, c
             const hipError_t e = saved.restore(b->acc, b->rng, b->fb);
             return r != ORN_OK ? r : from_hip(e);
         };
-        const orn_status r = coordinate_descent(start, flags, false, run, &best, &best_ms);
+        const orn_status r = coordinate_descent(start, flags, run, &best, &best_ms);
         (void)orn_batch_set_tuning(b, r == ORN_OK ? &best : &start);
         if (r != ORN_OK) return r;
         store_tuning(cache_path, key, best, best_ms);
