@@ -1059,19 +1059,28 @@ __global__ __launch_bounds__(kBlock) void nrc_paths_kernel(DevScene sc, orn_cons
 // random-spheres scene). Finely tessellated meshes send the rays of a tile
 // down different leaves, where a packet walks the union of their paths
 // (6x slower on the instanced-icosphere scene), so there each lane
-// traverses alone (LDS stack).
+// traverses alone (LDS stack). Workgroup b takes tile tile_order[b]: the
+// tiles whose slowest wave took longest in the previous launch go first, so
+// that the launch does not end on a costly tile dispatched last (camera
+// paths move little from frame to frame); each wave records its own time in
+// tile_cost for the next launch's order.
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_constant_params p,
                                                              uint32_t *__restrict__ rng_buf, PixelSink out,
                                                              NrcRecord *__restrict__ queries,
-                                                             uint32_t *__restrict__ query_count)
+                                                             uint32_t *__restrict__ query_count,
+                                                             const uint32_t *__restrict__ tile_order,
+                                                             uint32_t *__restrict__ tile_cost)
 {
     constexpr bool kPacket = !(F & kHasMeshes);
     extern __shared__ uint32_t lds_stack[];
+    const uint32_t t0 = device_ticks();
+    const uint32_t tile = tile_order[blockIdx.x];
     const uint32_t tiles_x = (p.width + 15u) / 16u;
+    if (tile >= tiles_x * ((p.height + 15u) / 16u)) return;  // (a permutation of the tiles: never)
     const uint32_t wave = threadIdx.x / 64u, lane = threadIdx.x % 64u;
-    const uint32_t x = (blockIdx.x % tiles_x) * 16u + (wave % 2u) * 8u + lane % 8u;
-    const uint32_t y = (blockIdx.x / tiles_x) * 16u + (wave / 2u) * 8u + lane / 8u;
+    const uint32_t x = (tile % tiles_x) * 16u + (wave % 2u) * 8u + lane % 8u;
+    const uint32_t y = (tile / tiles_x) * 16u + (wave / 2u) * 8u + lane / 8u;
     const bool active = x < p.width && y < p.height;
     const uint32_t pix = y * p.width + x;
     uint32_t rng = 0u;
@@ -1120,6 +1129,64 @@ __global__ __launch_bounds__(kBlock) void nrc_primary_kernel(DevScene sc, orn_co
     }
     const uint32_t slot = wave_append(query, query_count);
     if (query) queries[slot] = rc;
+    if (lane == 0u)
+        __hip_atomic_fetch_max(tile_cost + tile, device_ticks() - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The launch order of nrc_primary_kernel's tiles: a stable counting sort of
+// the tiles on the quarter-octave class of their recorded cost, costliest
+// first (tile order within a class; all-zero costs give the tile order),
+// by one workgroup, in chunks of 1,024 tiles; the costs are cleared for the
+// launch that records the next ones.
+constexpr uint32_t kOrderBlock = 1024;
+__global__ __launch_bounds__(kOrderBlock) void nrc_tile_order_kernel(uint32_t *__restrict__ cost, uint32_t tiles,
+                                                                     uint32_t *__restrict__ order)
+{
+    __shared__ uint32_t hist[kCostClasses];
+    __shared__ uint32_t base[kCostClasses];  // next position of each class
+    __shared__ uint32_t wave_count[kOrderBlock / 64u][kCostClasses];
+    const uint32_t tid = threadIdx.x, wave = tid / 64u, lane = tid % 64u;
+    if (tid < kCostClasses) hist[tid] = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < tiles; i += kOrderBlock) atomicAdd(&hist[cost_class(cost[i])], 1u);
+    __syncthreads();
+    if (tid < kCostClasses) {  // positions taken by the classes above
+        uint32_t above = 0;
+        for (uint32_t c = tid + 1u; c < kCostClasses; c++) above += hist[c];
+        base[tid] = above;
+    }
+    for (uint32_t c0 = 0; c0 < tiles; c0 += kOrderBlock) {
+        for (uint32_t i = tid; i < (kOrderBlock / 64u) * kCostClasses; i += kOrderBlock)
+            (&wave_count[0][0])[i] = 0u;
+        __syncthreads();
+        const uint32_t i = c0 + tid;
+        const bool valid = i < tiles;
+        const uint32_t cls = valid ? cost_class(cost[i]) : kCostClasses;
+        if (valid) cost[i] = 0u;
+        // rank among the wave's lanes of the same class, in lane order
+        uint32_t rank = 0;
+        uint64_t todo = lanes_where(valid);
+        while (todo != 0u) {
+            const uint32_t c = static_cast<uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(cls), __ffsll(static_cast<unsigned long long>(todo)) - 1));
+            const uint64_t m = lanes_where(cls == c);
+            if (cls == c) rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+            if (lane == 0u) wave_count[wave][c] = static_cast<uint32_t>(__popcll(m));
+            todo &= ~m;
+        }
+        __syncthreads();
+        if (valid) {
+            for (uint32_t w = 0; w < wave; w++) rank += wave_count[w][cls];
+            order[base[cls] + rank] = i;
+        }
+        __syncthreads();
+        if (tid < kCostClasses) {
+            uint32_t chunk = 0;
+            for (uint32_t w = 0; w < kOrderBlock / 64u; w++) chunk += wave_count[w][tid];
+            base[tid] += chunk;
+        }
+        __syncthreads();  // wave_count is cleared for the next chunk
+    }
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x)  // lowbias32 (Wellons)
@@ -1285,6 +1352,11 @@ struct orn_nrc {
     uint32_t train_cap;
     NrcRecord *queries;
     uint32_t query_cap;  // pixels the query buffer holds
+    // primary tiles queued costliest first (nrc_tile_order_kernel)
+    uint32_t *tile_cost;   // per tile: GPU time of its slowest wave in the latest primary launch
+    uint32_t *tile_order;  // launch order of the tiles
+    uint32_t tile_cap;     // tiles the two arrays hold
+    uint32_t tile_count;   // tiles of the image the costs were recorded for
     uint32_t *counters;  // [0] queries, [1] training records
     float *loss;         // [0] loss sum of the current step
     uint32_t step;       // Adam steps taken
@@ -1301,10 +1373,12 @@ void launch_paths(orn_nrc *n, const DevScene &ds, const orn_constant_params &p, 
                   hipStream_t st, size_t lds)
 {
     const uint32_t tiles = blocks_for(p.width, 16u) * blocks_for(p.height, 16u);
-    if (n->cfg.terminate_bounce == 0u)
+    if (n->cfg.terminate_bounce == 0u) {
+        nrc_tile_order_kernel<<<1, kOrderBlock, 0, st>>>(n->tile_cost, tiles, n->tile_order);
         nrc_primary_kernel<F><<<tiles, kBlock, (F & kHasMeshes) ? lds : 0, st>>>(ds, p, rng, out, n->queries,
-                                                                                n->counters);
-    else
+                                                                                n->counters, n->tile_order,
+                                                                                n->tile_cost);
+    } else
         nrc_paths_kernel<F><<<tiles, kBlock, lds, st>>>(ds, p, n->cfg.terminate_bounce, rng, out, n->queries,
                                                         n->counters);
 }
@@ -1339,7 +1413,7 @@ void free_nrc(orn_nrc *n)
     (void)hipStreamSynchronize(n->scene->ctx->stream);
     void *bufs[] = {n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, n->mlp_wt, n->grid_w, n->grid_m, n->grid_v,
                     n->grid_grad, n->grid_half, n->mlp_wt_alt, n->grid_half_alt, n->partials, n->train_recs,
-                    n->queries, n->counters,
+                    n->queries, n->counters, n->tile_cost, n->tile_order,
                     n->loss, n->sample_u, n->sample_dx, n->sample_mask, n->grid_scratch, n->train_stage, n->train_keep,
                     n->train_block_keep, n->train_offsets};
     for (void *b : bufs)
@@ -1721,6 +1795,22 @@ orn_status enqueue_render(orn_nrc *n, orn_target *t, const orn_constant_params *
         n->query_cap = 0;
         ORN_TRY(hipMalloc(&n->queries, size_t(px) * sizeof(NrcRecord)));
         n->query_cap = px;
+    }
+    const uint32_t tiles = blocks_for(t->width, 16u) * blocks_for(t->height, 16u);
+    if (tiles > n->tile_cap) {  // so do the tile arrays
+        ORN_TRY(hipStreamSynchronize(st));
+        if (n->tile_cost) (void)hipFree(n->tile_cost);
+        if (n->tile_order) (void)hipFree(n->tile_order);
+        n->tile_cost = n->tile_order = nullptr;
+        n->tile_cap = 0;
+        ORN_TRY(hipMalloc(&n->tile_cost, size_t(tiles) * sizeof(uint32_t)));
+        ORN_TRY(hipMalloc(&n->tile_order, size_t(tiles) * sizeof(uint32_t)));
+        n->tile_cap = tiles;
+        n->tile_count = 0;
+    }
+    if (tiles != n->tile_count) {  // costs of another tiling order nothing
+        nrc_zero_kernel<<<blocks_for(tiles, 256), 256, 0, st>>>(reinterpret_cast<float *>(n->tile_cost), tiles);
+        n->tile_count = tiles;
     }
     DevScene ds = s->ds;
     ds.stack_slots = s->stack_slots;

@@ -122,8 +122,6 @@ __device__ __forceinline__ void store_coherent(uint32_t *p, uint32_t v)
 //   a wavefront-scope acquire fence, which emits no instruction but keeps the
 //   compiler from moving the state loads above the stage-word load.
 __device__ __forceinline__ void release_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-// Low 32 bits of the device's constant 100 MHz clock (wave-uniform).
-__device__ __forceinline__ uint32_t device_ticks() { return static_cast<uint32_t>(__builtin_amdgcn_s_memrealtime()); }
 __device__ __forceinline__ void acquire_order() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
 
 // 16-byte accumulation entries go through a buffer descriptor, whose
@@ -655,18 +653,8 @@ __global__ void reset_queues_kernel(uint32_t *q)
 // (order_place_kernel), which also clears the costs for the render that
 // records the next ones. Pixels outside an image (partial edge tiles) take
 // class 0 and go last; the render skips them as before. With every cost zero
-// (nothing recorded yet) the order is the tile order.
-constexpr uint32_t kCostClasses = 128;
-
-// Quarter-octave class of a cost: 4 * floor(log2(c + 1)) plus the two bits
-// below the leading one of c + 1; non-decreasing in c, below 128.
-__device__ __forceinline__ uint32_t cost_class(uint32_t c)
-{
-    const uint32_t v = c == 0xffffffffu ? c : c + 1u;
-    const uint32_t e = 31u - static_cast<uint32_t>(__clz(v));
-    const uint32_t f = e >= 2u ? (v >> (e - 2u)) & 3u : (v << (2u - e)) & 3u;
-    return 4u * e + f;
-}
+// (nothing recorded yet) the order is the tile order. cost_class (the
+// quarter-octave classes) is in orn_trace.h.
 
 // The tiles a launch queues: one image's tiles in row-major order, cut into
 // kXcds bands of tiles_per_xcd (work == nullptr), or a batch's work list.

@@ -869,4 +869,19 @@ __device__ __forceinline__ uint32_t pack_bf16_rne(float a, float b)
     return __builtin_bit_cast(uint32_t, v);
 }
 
+// Low 32 bits of the device's constant 100 MHz clock (wave-uniform).
+__device__ __forceinline__ uint32_t device_ticks() { return static_cast<uint32_t>(__builtin_amdgcn_s_memrealtime()); }
+
+// Quarter-octave class of a cost (queues ordered by cost): 4 * floor(log2(c +
+// 1)) plus the two bits below the leading one of c + 1; non-decreasing in c,
+// below 128.
+constexpr uint32_t kCostClasses = 128;
+__device__ __forceinline__ uint32_t cost_class(uint32_t c)
+{
+    const uint32_t v = c == 0xffffffffu ? c : c + 1u;
+    const uint32_t e = 31u - static_cast<uint32_t>(__clz(v));
+    const uint32_t f = e >= 2u ? (v >> (e - 2u)) & 3u : (v << (2u - e)) & 3u;
+    return 4u * e + f;
+}
+
 }  // namespace orn
