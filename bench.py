@@ -16,7 +16,8 @@ trial renders. Tuning never changes the image. C2's tuning queues each XCD's
 pixels by what they cost in the target's previous render (cost_order: most
 expensive first, so the lanes run out of work nearly together); the warm-up
 renders record those costs, and `first_render_kernel_ms` reports the
-target's first render, which has none and queues in tile order.
+target's first render, which has none yet and records them in its first
+eighth of the samples (two launches).
 
 Also reported:
   roofline     the bound that limits the render kernel: vector-instruction
@@ -416,7 +417,7 @@ def main():
     for i in range(args.warmup):
         tgt.render(params, spec.spp)
         if i == 0:
-            first_ms = tgt.last_render_ms()  # no previous costs: tile-order queues
+            first_ms = tgt.last_render_ms()  # no previous costs yet
     tgt.wait()
     kernel_ms = []
     t0 = time.perf_counter()

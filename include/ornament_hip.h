@@ -260,9 +260,13 @@ orn_status orn_target_wait(orn_target *target);
  *   cost_order 1 (GPU time of their jobs, in quarter-octave classes; tile
  *   order within a class), so that the cheapest pixels are the last ones
  *   drawn and the end of the render, where lanes run out of work, is short
- *   (longest processing time first). The first such render queues in tile
- *   order and records the costs; every such render records them for the
- *   next. orn_render_noalloc ignores it (default 0). */
+ *   (longest processing time first). Every such render records the costs
+ *   for the next. A target's first one, with no costs recorded yet, renders
+ *   its first eighth of the samples in tile order to record them and the
+ *   rest queued by them (two launches with the same result as one; a batch's
+ *   first replay, or a render whose current_iteration is not a whole number
+ *   below 2^24 - iterations, queues in tile order). orn_render_noalloc
+ *   ignores it (default 0). */
 typedef struct orn_tuning {
     uint32_t regroup_lanes;
     uint32_t workgroups_per_cu;

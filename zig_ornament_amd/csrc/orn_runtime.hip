@@ -1148,12 +1148,21 @@ orn_tuning normalized(orn_tuning t)
 // is cut only when current_iteration + k stays exact in f32 for every k up
 // to its sample count (a whole number, below 2^24 at the end), and the job
 // counter of a band fits 31 bits.
+// Whether a render of `iterations` samples from current_iteration can be cut
+// (into jobs, or into launches) without changing a bit: the counter is a
+// whole number and stays below 2^24, where current_iteration + k is exact.
+bool splits_exactly(float current_iteration, uint32_t iterations)
+{
+    const uint32_t spp = iterations > 1u ? iterations : 1u;
+    return current_iteration >= 0.0f && std::floor(current_iteration) == current_iteration &&
+           double(current_iteration) + spp <= 16777216.0;
+}
+
 void plan_jobs(uint32_t iterations, float current_iteration, uint32_t samples_per_job, uint64_t band_pixels,
                uint32_t *chunk, uint32_t *chunks)
 {
     const uint32_t spp = iterations > 1u ? iterations : 1u;
-    const bool exact = current_iteration >= 0.0f && std::floor(current_iteration) == current_iteration &&
-                       double(current_iteration) + spp <= 16777216.0;
+    const bool exact = splits_exactly(current_iteration, iterations);
     uint32_t c = (samples_per_job == 0u || !exact) ? spp : std::min(samples_per_job, spp);
     while (true) {
         const uint32_t n = (spp + c - 1u) / c;
@@ -1704,10 +1713,13 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     ORN_TRY(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     const uint32_t n = t->width * t->height;
-    if (++t->epoch > 0xffffu) {  // tags wrap: clear the stage words
-        fill_u32_kernel<<<blocks_for(n, 256), 256, 0, st>>>(t->stage, n, 0u);
-        t->epoch = 1;
-    }
+    auto next_epoch = [&]() {
+        if (++t->epoch > 0xffffu) {  // tags wrap: clear the stage words
+            fill_u32_kernel<<<blocks_for(n, 256), 256, 0, st>>>(t->stage, n, 0u);
+            t->epoch = 1;
+        }
+    };
+    next_epoch();
     const bool by_cost = t->tuning.cost_order != 0u;
     if (by_cost) ORN_TRY(ensure_cost_order(t, st));
     ORN_TRY(hipEventRecord(t->ev_start, st));
@@ -1715,6 +1727,24 @@ orn_status orn_render_async(orn_target *t, const orn_constant_params *params, ui
     if (by_cost && t->costs_recorded) {
         enqueue_queue_order(t, st);
         order = t->order;
+    } else if (by_cost && splits_exactly(params->current_iteration, iterations) && iterations >= 2u) {
+        // No costs recorded yet: the first eighth of the samples runs in
+        // tile order and records them, and the rest, queued by those costs,
+        // continues it (a render of k then n - k samples from
+        // current_iteration + k gives the same bits as one of n).
+        const uint32_t first = std::max(1u, iterations / 8u);
+        ORN_TRY(enqueue_render(s, params, first, t->tuning, t->acc, t->rng, t->fb, t->stage, t->queues, t->epoch,
+                               st, nullptr, t->cost));
+        next_epoch();
+        enqueue_queue_order(t, st);
+        orn_constant_params rest = *params;
+        rest.current_iteration = params->current_iteration + static_cast<float>(first);
+        ORN_TRY(enqueue_render(s, &rest, iterations - first, t->tuning, t->acc, t->rng, t->fb, t->stage, t->queues,
+                               t->epoch, st, t->order, t->cost));
+        ORN_TRY(hipEventRecord(t->ev_stop, st));
+        t->timed = true;
+        t->costs_recorded = true;
+        return ORN_OK;
     }
     ORN_TRY(enqueue_render(s, params, iterations, t->tuning, t->acc, t->rng, t->fb, t->stage, t->queues, t->epoch,
                            st, order, by_cost ? t->cost : nullptr));
