@@ -164,7 +164,10 @@ def test_image_matches_oracle(device, name, tuning):
 
 @pytest.mark.parametrize("tuning", [dict(cost_order=1),
                                     dict(cost_order=1, samples_per_job=3, split_percent=100, regroup_lanes=8),
-                                    dict(cost_order=1, samples_per_job=2, split_percent=60, node_steps=6)])
+                                    dict(cost_order=1, samples_per_job=2, split_percent=60, node_steps=6),
+                                    dict(cost_order=1, bvh_width=4),
+                                    dict(cost_order=1, workgroup_pool=1, samples_per_job=3, split_percent=100,
+                                         regroup_lanes=8, workgroups_per_cu=1)])
 @pytest.mark.parametrize("name", ["C1", "C2s", "C3s", "C1tiny"])
 def test_cost_ordered_renders_match_oracle(device, name, tuning):
     """With cost_order 1 the first render records each pixel's cost and
